@@ -1,0 +1,134 @@
+/*
+ * mplb.h -- C ABI of the MI355X lattice-update library (libmplb.so).
+ *
+ * Drop-in replacement for the per-timestep path of the multiphaseLB solver: the time
+ * loop body of applications/solvers/multiPhasePP/multiPhasePP.c:157-315 and the
+ * routines it calls.  A case is described exactly as for the reference (the case
+ * directory's properties/{latticeProperties,macroProperties,simulation} and
+ * start/boundaries dictionaries, read with the reference's dictIO lookup rules);
+ * the lattice tables are those latticeBox2D/latticeBox3D + latticeMeshPartition
+ * (one rank) would write.  Plain C types only.  Every call returns 0 on success or
+ * a negative MPLB_E* code; mplb_last_error() gives the reference's message text
+ * where the reference would have called errorMsg() and exited.
+ *
+ * Threading: a case handle is used by one host thread at a time.  All calls are
+ * synchronous with respect to the host unless stated otherwise.
+ */
+#ifndef MPLB_H
+#define MPLB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPLB_OK 0
+#define MPLB_E_ARG (-1)        /* bad argument */
+#define MPLB_E_IO (-2)         /* a dictionary or file could not be read */
+#define MPLB_E_MODEL (-3)      /* the reference rejects this model/lattice/boundary combination */
+#define MPLB_E_HIP (-4)        /* HIP runtime error, or no usable GPU */
+#define MPLB_E_NAN (-5)        /* NaN found at a write step (writeScalarField.c:16-24) */
+#define MPLB_E_STATE (-6)      /* call out of order (e.g. step before init) */
+
+/* precision of the device state */
+#define MPLB_FP64 0            /* parity mode: reference arithmetic, bit-exact */
+#define MPLB_FP32 1
+#define MPLB_FP16S 2           /* populations stored as FP16 deviations f - w_k rho_ref, FP32 arithmetic */
+
+/* extension flags (behaviour the reference does not have; all off by default) */
+#define MPLB_EXT_LIMRT_D3Q15 0x1   /* accept Collision liMRT on D3Q15 as the xuMRT D3Q15 scheme */
+#define MPLB_EXT_NO_FIXEDGRADT_ERROR 0x2  /* treat fixedGradT as a no-op instead of the reference's fatal error */
+
+typedef struct mplb_opts {
+    int precision;         /* MPLB_FP64 / MPLB_FP32 / MPLB_FP16S */
+    int heat_transfer;     /* 1 = default, 0 = multiPhasePP --nht */
+    int frozen;            /* 0 = default, 1 = multiPhasePP --frozen */
+    int fused;             /* 1: use the fused structured-box kernels when the case allows it */
+    int extensions;        /* MPLB_EXT_* bits */
+    int device;            /* HIP device ordinal; -1 opens the case host-only (dictionaries and tables, no GPU) */
+} mplb_opts;
+
+typedef struct mplb_info {
+    int lbmodel;           /* 0 D2Q9, 1 D3Q15 */
+    int Q, d;
+    int nx, ny, nz;
+    int64_t npoints;       /* stored nodes (duplicate periodic end nodes included) */
+    int nbd;               /* number of boundary lists */
+    int f_model, g_model;  /* 0 liMRT, 1 myMRT, 2 xuMRT, 3 liTempMRT, -1 none */
+    int start_time, end_time, write_interval, current_time;
+    int precision;
+    int fused_active;      /* 1 if mplb_step runs the fused kernels */
+    double rho0;           /* EOS.rho_0 currently in use */
+} mplb_info;
+
+typedef struct mplb_case mplb_case;
+
+/* Default options (FP64, heat transfer on, not frozen, fused off, device 0). */
+void mplb_default_opts(mplb_opts* opts);
+
+/* Parse a reference case directory and build the lattice on the device.
+ * Replaces readLatticeMesh (IO/readLatticeMesh.c:101-660), setLatticeInfo
+ * (latticeModel/setLatticeInfo.c:16-90), readEOSInfo (EOS/readEOSInfo.c:10-111),
+ * readLbeField (IO/readLbeField.c:13-259) and latticeBox2D/latticeBox3D. */
+int mplb_case_open(const char* case_dir, const mplb_opts* opts, mplb_case** out);
+int mplb_case_close(mplb_case* c);
+const char* mplb_last_error(const mplb_case* c);
+int mplb_case_info(const mplb_case* c, mplb_info* out);
+
+/* Field transfer, double precision host arrays.  name: "rho", "T", "p", "U" (N x 3,
+ * node-major as the reference's U[id][j]), "Fi", "Ft", "f", "g" (N x Q, node-major),
+ * "ss" (g scalar source).  n is the number of doubles in the host buffer. */
+int mplb_set_field(mplb_case* c, const char* name, const double* host, size_t n);
+int mplb_get_field(mplb_case* c, const char* name, double* host, size_t n);
+
+/* Lattice tables: nb is N x Q node-major (nb[id*Q + k], upstream node x - c_k or -1),
+ * exactly the processor0/lattice/neighbours table of a one-rank case.
+ * mplb_get_boundary copies boundary list bid (file order) and its name. */
+int mplb_get_neighbours(const mplb_case* c, int32_t* nb, size_t n);
+int mplb_get_boundary(const mplb_case* c, int bid, int32_t* nodes, size_t n, char* name, size_t name_len);
+
+/* Solver initialisation, multiPhasePP.c:72-147: rho_0 = averageScalarField(rho),
+ * equilibrium(f), equilibrium(g), interForce, heatSource. */
+int mplb_init(mplb_case* c);
+
+/* Advance up to nsteps time steps of multiPhasePP.c:157-315 (stops at endTime).
+ * At write steps: macroPressure, rho_0 update and the NaN check (MPLB_E_NAN). */
+int mplb_step(mplb_case* c, int nsteps, int* steps_done);
+
+/* One routine of the step, for isolated parity checks; field 0 = f, 1 = g. */
+int mplb_collision(mplb_case* c, int field);              /* lbEquation/general/collision.c:6-67 */
+int mplb_stream(mplb_case* c, int field);                 /* lbEquation/general/lbstream.c:5-72 */
+int mplb_update_boundaries(mplb_case* c, int field);      /* lbEquation/general/updateBoundaries.c:6-75 */
+int mplb_macro_density(mplb_case* c);                     /* lbEquation/general/macroDensity.c:7-75 */
+int mplb_macro_velocity(mplb_case* c);                    /* lbEquation/general/macroVelocity.c:6-65 */
+int mplb_macro_temperature(mplb_case* c);                 /* lbEquation/general/macroTemperature.c:7-68 */
+int mplb_inter_force(mplb_case* c);                       /* lbEquation/pseudoPot/interForce.c:7-52 */
+int mplb_heat_source(mplb_case* c);                       /* lbEquation/general/heatSource.c:7-202 */
+int mplb_equilibrium(mplb_case* c, int field);            /* lbEquation/general/equilibrium.c:6-90 */
+int mplb_macro_pressure(mplb_case* c);                    /* lbEquation/general/macroPressure.c:8-69 */
+/* basic/averageScalarField.c:4-78 on rho (0 when rho_0 <= 0, as the reference). */
+int mplb_average_density(mplb_case* c, double* out);
+
+/* Timing: runs nsteps steps and reports the elapsed device time of the whole region
+ * and of each kernel class measured with HIP events on the library's stream.
+ * kernel_ms has room for MPLB_NTIMERS doubles (average ms per launch), names via
+ * mplb_timer_name(). */
+#define MPLB_NTIMERS 8
+int mplb_time_steps(mplb_case* c, int nsteps, double* total_ms, double* kernel_ms, int* kernel_launches);
+/* (returns the number of steps taken, or a negative MPLB_E* code) */
+const char* mplb_timer_name(int i);
+
+/* Constant tables of the lattice models (for the bit-exact table check). */
+int mplb_lattice_tables(int lbmodel, int* vel, int* rev, double* omega, double* w, double* M, double* invM);
+
+/* glibc-compatible rand() stream used to precompute the fixedRandomT draws
+ * (fixedRandomT.c:37,81): writes n successive values after srand(seed). */
+int mplb_glibc_rand(unsigned seed, int n, int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,251 @@
+"""multiphaselb_amd -- MI355X lattice update for multiphaseLB cases.
+
+Thin ctypes layer over libmplb.so (include/mplb.h).  All compute runs in the HIP
+library; this module only moves arrays and writes case dictionaries.  Importing it
+does not need a GPU; opening a case does, and fails loudly when the library is not
+built or no GPU is usable.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .casefiles import CaseSpec, write_case  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmplb.so")
+
+FP64, FP32, FP16S = 0, 1, 2
+EXT_LIMRT_D3Q15 = 0x1
+EXT_NO_FIXEDGRADT_ERROR = 0x2
+MODEL_NAMES = {0: "liMRT", 1: "myMRT", 2: "xuMRT", 3: "liTempMRT", -1: None}
+NTIMERS = 8
+
+ERRORS = {-1: "argument", -2: "io", -3: "model", -4: "hip", -5: "nan", -6: "state"}
+
+
+class MplbError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("mplb error %d (%s): %s" % (code, ERRORS.get(code, "?"), msg))
+        self.code = code
+        self.msg = msg
+
+
+class Opts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("heat_transfer", C.c_int), ("frozen", C.c_int),
+                ("fused", C.c_int), ("extensions", C.c_int), ("device", C.c_int)]
+
+
+class Info(C.Structure):
+    _fields_ = [("lbmodel", C.c_int), ("Q", C.c_int), ("d", C.c_int), ("nx", C.c_int),
+                ("ny", C.c_int), ("nz", C.c_int), ("npoints", C.c_int64), ("nbd", C.c_int),
+                ("f_model", C.c_int), ("g_model", C.c_int), ("start_time", C.c_int),
+                ("end_time", C.c_int), ("write_interval", C.c_int), ("current_time", C.c_int),
+                ("precision", C.c_int), ("fused_active", C.c_int), ("rho0", C.c_double)]
+
+
+_lib = None
+
+EXPORTS = [
+    "mplb_default_opts", "mplb_case_open", "mplb_case_close", "mplb_last_error", "mplb_case_info",
+    "mplb_set_field", "mplb_get_field", "mplb_get_neighbours", "mplb_get_boundary", "mplb_init",
+    "mplb_step", "mplb_collision", "mplb_stream", "mplb_update_boundaries", "mplb_macro_density",
+    "mplb_macro_velocity", "mplb_macro_temperature", "mplb_inter_force", "mplb_heat_source",
+    "mplb_equilibrium", "mplb_macro_pressure", "mplb_average_density", "mplb_time_steps",
+    "mplb_timer_name", "mplb_lattice_tables", "mplb_glibc_rand",
+]
+
+
+def library():
+    """Load libmplb.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libmplb.so is not built; run `python -m multiphaselb_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, cp, i, d = C.c_void_p, C.c_char_p, C.c_int, C.c_double
+    sig = {
+        "mplb_default_opts": ([C.POINTER(Opts)], None),
+        "mplb_case_open": ([cp, C.POINTER(Opts), C.POINTER(vp)], i),
+        "mplb_case_close": ([vp], i),
+        "mplb_last_error": ([vp], cp),
+        "mplb_case_info": ([vp, C.POINTER(Info)], i),
+        "mplb_set_field": ([vp, cp, vp, C.c_size_t], i),
+        "mplb_get_field": ([vp, cp, vp, C.c_size_t], i),
+        "mplb_get_neighbours": ([vp, vp, C.c_size_t], i),
+        "mplb_get_boundary": ([vp, i, vp, C.c_size_t, cp, C.c_size_t], i),
+        "mplb_init": ([vp], i),
+        "mplb_step": ([vp, i, C.POINTER(i)], i),
+        "mplb_collision": ([vp, i], i),
+        "mplb_stream": ([vp, i], i),
+        "mplb_update_boundaries": ([vp, i], i),
+        "mplb_equilibrium": ([vp, i], i),
+        "mplb_macro_density": ([vp], i),
+        "mplb_macro_velocity": ([vp], i),
+        "mplb_macro_temperature": ([vp], i),
+        "mplb_inter_force": ([vp], i),
+        "mplb_heat_source": ([vp], i),
+        "mplb_macro_pressure": ([vp], i),
+        "mplb_average_density": ([vp, C.POINTER(d)], i),
+        "mplb_time_steps": ([vp, i, C.POINTER(d), vp, vp], i),
+        "mplb_timer_name": ([i], cp),
+        "mplb_lattice_tables": ([i, vp, vp, vp, vp, vp, vp], i),
+        "mplb_glibc_rand": ([C.c_uint, i, vp], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def lattice_tables(lbmodel):
+    Q = 9 if lbmodel == 0 else 15
+    vel = np.zeros((Q, 3), np.int32)
+    rev = np.zeros(Q, np.int32)
+    omega, w = np.zeros(Q), np.zeros(Q)
+    M, invM = np.zeros((Q, Q)), np.zeros((Q, Q))
+    library().mplb_lattice_tables(lbmodel, vel.ctypes.data, rev.ctypes.data, omega.ctypes.data,
+                                  w.ctypes.data, M.ctypes.data, invM.ctypes.data)
+    return dict(vel=vel, rev=rev, omega=omega, w=w, M=M, invM=invM)
+
+
+def glibc_rand(seed, n):
+    out = np.zeros(n, np.int32)
+    library().mplb_glibc_rand(seed, n, out.ctypes.data)
+    return out
+
+
+class Case:
+    """One case on the GPU (mplb_case_open / mplb_case_close)."""
+
+    def __init__(self, case_dir, precision=FP64, heat_transfer=True, frozen=False, fused=False,
+                 extensions=0, device=0):
+        self.L = library()
+        o = Opts()
+        self.L.mplb_default_opts(C.byref(o))
+        o.precision = precision
+        o.heat_transfer = 1 if heat_transfer else 0
+        o.frozen = 1 if frozen else 0
+        o.fused = 1 if fused else 0
+        o.extensions = extensions
+        o.device = device
+        h = C.c_void_p()
+        rc = self.L.mplb_case_open(str(case_dir).encode(), C.byref(o), C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.L.mplb_last_error(h).decode() if h.value else "open failed"
+            self.close()
+            raise MplbError(rc, msg)
+        self.info = self.get_info()
+        self.N = self.info.npoints
+        self.Q = self.info.Q
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.mplb_case_close(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise MplbError(rc, self.L.mplb_last_error(self.h).decode())
+        return rc
+
+    def get_info(self):
+        inf = Info()
+        self._check(self.L.mplb_case_info(self.h, C.byref(inf)))
+        return inf
+
+    def _shape(self, name):
+        if name in ("rho", "T", "p", "ss"):
+            return (self.N,)
+        if name in ("U", "Fi", "Ft"):
+            return (self.N, 3)
+        return (self.N, self.Q)
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self._shape(name))
+        self._check(self.L.mplb_set_field(self.h, name.encode(), a.ctypes.data, a.size))
+
+    def get(self, name):
+        a = np.zeros(self._shape(name))
+        self._check(self.L.mplb_get_field(self.h, name.encode(), a.ctypes.data, a.size))
+        return a
+
+    def neighbours(self):
+        a = np.zeros((self.N, self.Q), np.int32)
+        self._check(self.L.mplb_get_neighbours(self.h, a.ctypes.data, a.size))
+        return a
+
+    def boundaries(self):
+        out = []
+        for b in range(self.info.nbd):
+            n = self.L.mplb_get_boundary(self.h, b, None, 0, None, 0)
+            a = np.zeros(n, np.int32)
+            name = C.create_string_buffer(64)
+            self._check(self.L.mplb_get_boundary(self.h, b, a.ctypes.data, n, name, 64))
+            out.append((name.value.decode(), a))
+        return out
+
+    def init(self):
+        self._check(self.L.mplb_init(self.h))
+
+    def step(self, n=1):
+        done = C.c_int(0)
+        self._check(self.L.mplb_step(self.h, n, C.byref(done)))
+        return done.value
+
+    def average_density(self):
+        v = C.c_double()
+        self._check(self.L.mplb_average_density(self.h, C.byref(v)))
+        return v.value
+
+    def time_steps(self, n, per_kernel=True):
+        tot = C.c_double()
+        kms = np.zeros(NTIMERS)
+        kn = np.zeros(NTIMERS, np.int32)
+        done = self._check(self.L.mplb_time_steps(self.h, n, C.byref(tot),
+                                                   kms.ctypes.data if per_kernel else None,
+                                                   kn.ctypes.data if per_kernel else None))
+        names = [self.L.mplb_timer_name(i).decode() for i in range(NTIMERS)]
+        kern = {names[i]: (float(kms[i]), int(kn[i])) for i in range(NTIMERS) if kn[i] > 0}
+        return done, tot.value, kern
+
+    # fine-grained routines (field 0 = f, 1 = g)
+    def collision(self, field):
+        self._check(self.L.mplb_collision(self.h, field))
+
+    def stream(self, field):
+        self._check(self.L.mplb_stream(self.h, field))
+
+    def update_boundaries(self, field):
+        self._check(self.L.mplb_update_boundaries(self.h, field))
+
+    def equilibrium(self, field):
+        self._check(self.L.mplb_equilibrium(self.h, field))
+
+    def macro_density(self):
+        self._check(self.L.mplb_macro_density(self.h))
+
+    def macro_velocity(self):
+        self._check(self.L.mplb_macro_velocity(self.h))
+
+    def macro_temperature(self):
+        self._check(self.L.mplb_macro_temperature(self.h))
+
+    def inter_force(self):
+        self._check(self.L.mplb_inter_force(self.h))
+
+    def heat_source(self):
+        self._check(self.L.mplb_heat_source(self.h))
+
+    def macro_pressure(self):
+        self._check(self.L.mplb_macro_pressure(self.h))

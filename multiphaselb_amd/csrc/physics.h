@@ -1,0 +1,153 @@
+// Per-node physics shared by the HIP kernels: equation of state, pseudopotential,
+// equilibria and the MRT moment transforms.  Every expression keeps the reference's
+// association order (plain IEEE arithmetic; the FP64 translation unit is built with
+// -ffp-contract=off so no multiply-add is fused), which is what makes the FP64 mode
+// reproduce the reference results bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lattice.h"
+
+namespace mplb {
+
+enum EosType : int { EOS_IDEAL = 0, EOS_CARSTAR = 1, EOS_PENGROB = 2, EOS_VDW = 3 };
+
+// Scalars of the EOS block (EOS/EOSInfo.h:80-102).
+struct EosParams {
+    int type;
+    double a, b, w_pr, R, G, M, Cv;
+    double g[3], fext[3];
+};
+
+// EOS/p_eos.c:7-64
+template <typename T>
+__device__ __host__ inline T p_eos(const EosParams& e, T rho, T temp) {
+    T p = 0, alpha, beta;
+    const T R = T(e.R), a = T(e.a), b = T(e.b), Mm = T(e.M);
+    switch (e.type) {
+    case EOS_IDEAL:
+        p = rho * R * temp;
+        break;
+    case EOS_CARSTAR:
+        alpha = b * rho / T(4.0);
+        beta = T(1) - alpha;
+        p = rho * R * temp * ((T(1) + alpha + alpha * alpha - alpha * alpha * alpha) / (beta * beta * beta)) -
+            a * rho * rho;
+        break;
+    case EOS_PENGROB: {
+        const T w = T(e.w_pr);
+        alpha = T(0.170151343) * a / (b * R);
+        beta = (T(1) + (T(0.37464) + T(1.5422) * w - T(0.26992) * w * w) * (T(1) - sqrt(temp / alpha)));
+        p = rho * R * temp / (T(1) - b * rho) -
+            a * beta * beta * rho * rho / (T(1) + T(2) * b * rho - b * rho * b * rho);
+        break;
+    }
+    case EOS_VDW:
+        p = rho * R * temp / (Mm - rho * b) - a * rho * rho / (Mm * Mm);
+        break;
+    default:
+        break;
+    }
+    return p;
+}
+
+// lbEquation/pseudoPot/potential.c:7-13
+template <typename T>
+__device__ __host__ inline T potential(const EosParams& e, T rho, T temp) {
+    return sqrt(T(2) * (p_eos<T>(e, rho, temp) - rho * T(kCs2)) / T(e.G));
+}
+
+// res = M v, j ascending from a +0 start; zero coefficients are skipped (adding the
+// reference's +-0 products never changes a running sum) and the power-of-two
+// coefficients multiply exactly, so the rounding sequence equals the reference's dense
+// matVecMult (basic/matVecMult.c:4-20).
+template <class L, typename T>
+__device__ __host__ inline void moments(const T* v, T* res) {
+#pragma unroll
+    for (int i = 0; i < L::Q; i++) {
+        T r = T(0);
+#pragma unroll
+        for (int j = 0; j < L::Q; j++) {
+            const int m = L::t.M[i][j];
+            if (m == 0) continue;
+            r += (m == 1) ? v[j] : (m == -1 ? -v[j] : T(m) * v[j]);
+        }
+        res[i] = r;
+    }
+}
+
+// res = invM v, j ascending, zero coefficients skipped.
+template <class L, typename T>
+__device__ __host__ inline void populations(const T* v, T* res) {
+#pragma unroll
+    for (int i = 0; i < L::Q; i++) {
+        T r = T(0);
+#pragma unroll
+        for (int j = 0; j < L::Q; j++) {
+            const double m = L::t.invM[i][j];
+            if (m == 0.0) continue;
+            r += T(m) * v[j];
+        }
+        res[i] = r;
+    }
+}
+
+// liMRTEquilibrium.c:3-29 / xuMRTEquilibrium.c:3-29, population k only.
+template <class L, typename T>
+__device__ __host__ inline T ns_feq(int k, T rho, const T v[3]) {
+    T alpha = 0, beta = 0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int c = L::t.c[k][j];
+        if (c == 1) alpha += v[j];
+        else if (c == -1) alpha += -v[j];
+        beta += v[j] * v[j];
+    }
+    const T cs2 = T(kCs2);
+    return rho * T(L::t.omega[k]) *
+           (T(1) + alpha / cs2 + T(0.5) * alpha * alpha / (cs2 * cs2) - T(0.5) * beta / cs2);
+}
+
+// myMRTEquilibrium.c:5-95: g_eq = invM n_eq.
+template <class L, typename T>
+__device__ __host__ inline void my_geq(T temp, const T U[3], T a1, T a2, T* geq) {
+    T n[L::Q];
+#pragma unroll
+    for (int k = 0; k < L::Q; k++) n[k] = T(0);
+    n[0] = temp;
+    n[1] = a1 * temp;
+    n[2] = a2 * temp;
+    if (L::D == 2) {
+        n[3] = temp * U[0];
+        n[4] = temp * (-U[0]);
+        n[5] = temp * U[1];
+        n[6] = temp * (-U[1]);
+    } else {
+        n[3] = temp * U[0];
+        n[4] = -(temp * U[0]);
+        n[5] = temp * U[1];
+        n[6] = -(temp * U[1]);
+        n[7] = temp * U[2];
+        n[8] = -(temp * U[2]);
+    }
+    populations<L, T>(n, geq);
+}
+
+// liTempMRTEquilibrium.c:5-44 (D2Q9 only).
+template <class L, typename T>
+__device__ __host__ inline void litemp_geq(T temp, const T U[3], T* geq) {
+    T n[L::Q];
+#pragma unroll
+    for (int k = 0; k < L::Q; k++) n[k] = T(0);
+    n[0] = temp;
+    n[1] = T(-2.0) * temp;
+    n[2] = T(2.0) * temp;
+    n[3] = temp * U[0];
+    n[4] = temp * (-U[0]);
+    n[5] = temp * U[1];
+    n[6] = temp * (-U[1]);
+    populations<L, T>(n, geq);
+}
+
+}  // namespace mplb
