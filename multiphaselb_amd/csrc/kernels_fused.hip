@@ -1,22 +1,1087 @@
-// Fused structured-box kernels (placeholder until the fused path lands).
+// Fused structured-box path: FP32 arithmetic, FP32 or FP16-deviation population storage.
+//
+// The state between steps is the post-collision population set of the current step.
+// One time step is two node-parallel passes (three with heat transfer):
+//   K1  pull-stream f (and g) from the post-collision buffers, apply the boundary
+//       conditions of the node in registers, and write what the neighbours' stencils
+//       need: psi = potential(rho, T) (isothermal) or rho and sum g (thermal);
+//   KH  (thermal) heat source from the rho / T / U stencils, new T, psi;
+//   K2  pull again (the same addresses K1 touched, mostly served by the caches), psi
+//       stencil -> interaction force -> velocity, then the MRT collisions of the next
+//       step, written to the other buffer.
+// Interior nodes compute the pull source as a fixed offset; nodes on the box surface
+// read the neighbour table, so the reference's duplicate-node periodic links and wall
+// links are followed exactly.  Blocks are numbered so that each XCD owns a contiguous
+// range of z planes and the planes a block pulls from share that XCD's L2.
 #include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
 
 #include "case.h"
 #include "fused.h"
+#include "physics.h"
 
 namespace mplb {
 
-bool fused_supported(const mplb_case*, std::string* why) {
-    if (why) *why = "not built";
-    return false;
+namespace {
+
+enum : uint8_t { FB_NONE = 0, FB_BBACK = 1, FB_FT = 2, FB_FRDT = 3 };
+
+struct FArgs {
+    int nx, ny, nz;
+    int64_t N, nxy;
+    int64_t P;                                  // pitch between population planes (>= N)
+    int nblocks;                                // 256-node blocks
+    int face_table;                             // faces (x0 x1 y0 y1 z0 z1 bits) read through the table
+    const int* nb;                              // [Q][N]
+    const uint8_t* onbnd;
+    const uint8_t* bcf;                         // per-node boundary code, f (surface nodes)
+    const uint8_t* bcg;
+    const float* bcTw;                          // per-node wall temperature (g)
+    const float* bcPert;                        // per-node fixedRandomT percentage
+    const int* rnd_off;                         // per-node first draw
+    const double* rnd;
+    int draw;                                   // fixedRandomT drawing this step
+    EosParams eos;
+    float rho0;
+    int fmodel, gmodel, surf;
+    float fl[15], gl[15];
+    float sigma, kappa, a1, a2;
+    const void* fin;
+    void* fout;
+    const void* gin;
+    void* gout;
+    float* rho;
+    float* sumg;
+    float* psi;
+    float* S;
+    const float* Told;
+    float* Tnew;
+    const float* Uold;
+    float* Unew;
+    float Tconst;
+    int Tuniform;
+    // optional outputs (materialisation)
+    float* out_rho;
+    float* out_U;
+    float* out_Fi;
+    float* out_Ft;
+    int first_call;                             // liTempMRT heat source S_old = 0
+    float fshift[15], gshift[15];               // FP16 deviation storage shifts
+};
+
+template <typename S>
+__device__ inline float ld(const void* p, int64_t i, float shift) {
+    if constexpr (sizeof(S) == 2) {
+        return (float)((const _Float16*)p)[i] + shift;
+    } else {
+        return ((const float*)p)[i];
+    }
 }
-int fused_create(mplb_case*) { return MPLB_E_STATE; }
-void fused_destroy(mplb_case*) {}
-int fused_after_init(mplb_case*) { return 0; }
-int fused_step(mplb_case*) { return MPLB_E_STATE; }
-int fused_sync_to_generic(mplb_case*) { return 0; }
-int fused_after_write(mplb_case*) { return 0; }
-int fused_set_field(mplb_case*, const char*, const double*, size_t) { return 1; }
-int fused_get_field(mplb_case*, const char*, double*, size_t) { return 1; }
+
+#ifndef MPLB_NT_STORE
+#define MPLB_NT_STORE 1
+#endif
+
+template <typename S>
+__device__ inline void st(void* p, int64_t i, float v, float shift) {
+    if constexpr (sizeof(S) == 2) {
+        if (MPLB_NT_STORE) __builtin_nontemporal_store((_Float16)(v - shift), (_Float16*)p + i);
+        else ((_Float16*)p)[i] = (_Float16)(v - shift);
+    } else {
+        if (MPLB_NT_STORE) __builtin_nontemporal_store(v, (float*)p + i);
+        else ((float*)p)[i] = v;
+    }
+}
+
+struct NodePos {
+    int i, j, k;
+    int64_t id;
+    bool surface;       // on the box surface
+    bool table;         // on a face whose links must be read from the neighbour table
+    bool valid;
+    unsigned X[3], Y[3], Z[3];   // wrapped coordinates u-1, u, u+1 premultiplied by the strides
+};
+
+// Block b covers the 256 consecutive node ids [256 b, 256 b + 256).
+template <class L>
+__device__ inline NodePos node_pos(const FArgs& a) {
+    NodePos p;
+    const unsigned id = blockIdx.x * 256u + threadIdx.x;     // N < 2^31
+    const unsigned nxy = (unsigned)a.nxy, nx = (unsigned)a.nx;
+    p.id = id;
+    p.valid = p.id < a.N;
+    const unsigned k = id / nxy, r = id - k * nxy, j = r / nx;
+    p.k = (int)k;
+    p.j = (int)j;
+    p.i = (int)(r - j * nx);
+    const bool x0 = p.i == 0, x1 = p.i == a.nx - 1, y0 = p.j == 0, y1 = p.j == a.ny - 1;
+    const bool z0 = L::D == 3 && p.k == 0, z1 = L::D == 3 && p.k == a.nz - 1;
+    p.surface = x0 || x1 || y0 || y1 || z0 || z1;
+    const int m = a.face_table;
+    p.table = (x0 && (m & 1)) || (x1 && (m & 2)) || (y0 && (m & 4)) || (y1 && (m & 8)) || (z0 && (m & 16)) ||
+              (z1 && (m & 32));
+    // periodic wrap with duplicate end nodes: u - 1 at 0 is n - 2, u + 1 at n - 1 is 1
+    p.X[0] = x0 ? nx - 2 : p.i - 1;
+    p.X[1] = p.i;
+    p.X[2] = x1 ? 1 : p.i + 1;
+    p.Y[0] = (y0 ? a.ny - 2 : p.j - 1) * nx;
+    p.Y[1] = j * nx;
+    p.Y[2] = (y1 ? 1 : p.j + 1) * nx;
+    if (L::D == 3) {
+        p.Z[0] = (z0 ? a.nz - 2 : p.k - 1) * nxy;
+        p.Z[1] = k * nxy;
+        p.Z[2] = (z1 ? 1 : p.k + 1) * nxy;
+    } else {
+        p.Z[0] = p.Z[1] = p.Z[2] = 0;
+    }
+    return p;
+}
+
+// upstream node x - c_q (lbstream.c:16-62: a missing link keeps the node's own value)
+template <class L>
+__device__ inline unsigned upstream(const FArgs& a, const NodePos& p, int q) {
+    if (p.table) {
+        const int s = a.nb[q * a.N + p.id];
+        return s == -1 ? (unsigned)p.id : (unsigned)s;
+    }
+    return p.X[1 - L::t.c[q][0]] + p.Y[1 - L::t.c[q][1]] + p.Z[1 - L::t.c[q][2]];
+}
+
+// downstream node x + c_q (the interaction-force and stencil neighbour), -1 if missing
+template <class L>
+__device__ inline int64_t downstream(const FArgs& a, const NodePos& p, int q) {
+    if (p.table) return a.nb[L::t.rev[q] * a.N + p.id];
+    return (int64_t)(p.X[1 + L::t.c[q][0]] + p.Y[1 + L::t.c[q][1]] + p.Z[1 + L::t.c[q][2]]);
+}
+
+template <class L, typename S>
+__device__ inline void pull(const FArgs& a, const NodePos& p, const void* src, const float* shift, float* v) {
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) {
+        const void* plane = (const char*)src + (size_t)q * a.P * sizeof(S);
+        v[q] = ld<S>(plane, upstream<L>(a, p, q), shift[q]);
+    }
+}
+
+// bounceBack.c:14-75 in registers
+template <class L>
+__device__ inline void bounce_back(const FArgs& a, const NodePos& p, float* v) {
+    bool miss[L::Q];
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) miss[q] = a.nb[q * a.N + p.id] == -1;
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) {
+        const int r = L::t.rev[q];
+        if (miss[q]) {
+            v[q] = v[r];
+        } else if (!miss[r] && q >= 1 && q <= L::naxis) {
+            v[q] = 0.5f * v[q] + 0.5f * v[r];
+            v[r] = v[q];
+        }
+    }
+}
+
+// fixedT.c:45-122 / fixedRandomT.c:48-112 in registers; the neighbour's post-stream
+// population is pulled directly and its T and U are the previous step's.
+template <class L, typename S>
+__device__ inline void fixed_temperature(const FArgs& a, const NodePos& p, float* v, uint8_t code) {
+    int draw = code == FB_FRDT ? a.rnd_off[p.id] : 0;
+    const float Tw0 = a.bcTw[p.id];
+    float minT = 0, maxT = 0;
+    if (code == FB_FRDT) {
+        const float pert = a.bcPert[p.id];
+        minT = Tw0 * (100.0f - pert) / 100.0f;
+        maxT = Tw0 * (100.0f + pert) / 100.0f;
+    }
+    for (int q = 0; q < L::Q; q++) {
+        if (a.nb[q * a.N + p.id] != -1) continue;
+        const int nbid = a.nb[L::t.rev[q] * a.N + p.id];
+        if (nbid == -1) continue;
+        const int s2 = a.nb[q * a.N + nbid];
+        const float gnb = ld<S>(a.gin, q * a.P + (s2 == -1 ? (int64_t)nbid : (int64_t)s2), a.gshift[q]);
+        const float U[3] = {a.Uold[nbid], a.Uold[a.N + nbid], a.Uold[2 * a.N + nbid]};
+        const float Tnb = a.Tuniform ? a.Tconst : a.Told[nbid];
+        float geq[L::Q];
+        if (a.gmodel == 1) my_geq<L, float>(Tnb, U, a.a1, a.a2, geq);
+        else litemp_geq<L, float>(Tnb, U, geq);
+        const float eq_nb = geq[q];
+        float Tw = Tw0;
+        if (code == FB_FRDT && a.draw) Tw = minT + (maxT - minT) * (float)a.rnd[draw++];
+        if (a.gmodel == 1) my_geq<L, float>(Tw, U, a.a1, a.a2, geq);
+        else litemp_geq<L, float>(Tw, U, geq);
+        v[q] = geq[q] + (gnb - eq_nb);
+    }
+}
+
+template <class L, typename S>
+__device__ inline void pull_f(const FArgs& a, const NodePos& p, float* v) {
+    pull<L, S>(a, p, a.fin, a.fshift, v);
+    if (p.table && a.bcf[p.id] == FB_BBACK) bounce_back<L>(a, p, v);
+}
+
+template <class L, typename S>
+__device__ inline void pull_g(const FArgs& a, const NodePos& p, float* v) {
+    pull<L, S>(a, p, a.gin, a.gshift, v);
+    if (!p.table) return;
+    const uint8_t code = a.bcg[p.id];
+    if (code == FB_BBACK) bounce_back<L>(a, p, v);
+    else if (code == FB_FT || code == FB_FRDT) fixed_temperature<L, S>(a, p, v, code);
+}
+
+// K1
+template <class L, typename S, bool THERMAL>
+__global__ void __launch_bounds__(256) k_fused_density(FArgs a) {
+    const NodePos p = node_pos<L>(a);
+    if (!p.valid) return;
+    float v[L::Q];
+    pull_f<L, S>(a, p, v);
+    float rho = 0.f;
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) rho += v[q];
+    if (THERMAL) {
+        pull_g<L, S>(a, p, v);
+        float sg = 0.f;
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) sg += v[q];
+        a.rho[p.id] = rho;
+        a.sumg[p.id] = sg;
+    } else {
+        const float T = a.Tuniform ? a.Tconst : a.Told[p.id];
+        a.psi[p.id] = potential<float>(a.eos, rho, T);
+        if (a.out_rho) a.out_rho[p.id] = rho;
+    }
+}
+
+// KH: heatSource.c:26-170 with macroTemperature (myMRTTemperature.c:16-30)
+template <class L>
+__global__ void __launch_bounds__(256) k_fused_heat(FArgs a) {
+    const NodePos p = node_pos<L>(a);
+    if (!p.valid) return;
+    const int64_t id = p.id;
+    const float rho = a.rho[id], Told = a.Told[id];
+    const float Cv = (float)a.eos.Cv;
+    float lambda;
+    if (a.gmodel == 1) {
+        lambda = (L::D == 2) ? rho * Cv * (1.0f / a.gl[3] - 0.5f) * (4.0f + 3.0f * a.a1 + 2.0f * a.a2) / 6.0f
+                             : rho * Cv * (1.0f / a.gl[3] - 0.5f) * (6.0f + 11.0f * a.a1 + a.a2) / 9.0f;
+    } else {
+        lambda = rho * Cv * (1.0f / a.gl[3] - 0.5f);
+    }
+    float dot = 0.f, divU = 0.f;
+#pragma unroll
+    for (int ax = 0; ax < L::D; ax++) {
+        // a = nb[id][fd_plus] = x + e_ax, b = nb[id][fd_minus] = x - e_ax
+        const int64_t na = downstream<L>(a, p, L::fd_minus[ax]);
+        const int64_t nbb = downstream<L>(a, p, L::fd_plus[ax]);
+        float gT, gR, dU;
+        const float* Uax = a.Uold + ax * a.N;
+        if (na != -1 && nbb != -1) {
+            gT = 0.5f * (a.Told[na] - a.Told[nbb]);
+            gR = 0.5f * (1.0f / a.rho[na] - 1.0f / a.rho[nbb]);
+            dU = 0.5f * (Uax[na] - Uax[nbb]);
+        } else if (na == -1 && nbb != -1) {
+            gT = Told - a.Told[nbb];
+            gR = 1.0f / rho - 1.0f / a.rho[nbb];
+            dU = Uax[id] - Uax[nbb];
+        } else if (na != -1 && nbb == -1) {
+            gT = a.Told[na] - Told;
+            gR = 1.0f / a.rho[na] - 1.0f / rho;
+            dU = Uax[na] - Uax[id];
+        } else {
+            gT = 0.f;
+            gR = 0.f;
+            dU = 0.f;
+        }
+        dot += gT * gR;
+        divU += dU;
+    }
+    float dpdT = p_eos<float>(a.eos, rho, 1.1f * Told) - p_eos<float>(a.eos, rho, 0.9f * Told);
+    dpdT = dpdT / (0.2f * Told);
+    const float s = -lambda * dot / Cv + divU * Told * (1.0f - dpdT / (rho * Cv));
+    float T;
+    float Sv;
+    if (a.gmodel == 1) {
+        Sv = s;
+        T = a.sumg[id] + 0.5f * Sv;
+    } else {
+        const float old = a.first_call ? 0.f : a.S[id];
+        Sv = s + 0.5f * (s - old);
+        T = a.sumg[id];
+    }
+    a.S[id] = Sv;
+    a.Tnew[id] = T;
+    a.psi[id] = potential<float>(a.eos, rho, T);
+}
+
+// K2
+template <class L, typename S, bool THERMAL>
+__global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
+    const NodePos p = node_pos<L>(a);
+    if (!p.valid) return;
+    const int64_t id = p.id, N = a.N;
+    float v[L::Q];
+    pull_f<L, S>(a, p, v);
+    float rho = 0.f;
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) rho += v[q];
+    // interaction force (interactionForce.c:47-79)
+    const float psic = a.psi[id];
+    const bool interior = !p.table || a.onbnd[id] == 0;
+    float Fi[3] = {0.f, 0.f, 0.f};
+    float psin[L::Q];
+    psin[0] = psic;
+    if (interior || a.surf) {
+#pragma unroll
+        for (int q = 1; q < L::Q; q++) {
+            const int64_t n = downstream<L>(a, p, q);
+            psin[q] = n == -1 ? psic : a.psi[n];
+        }
+    }
+    if (interior) {
+        float F[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 1; q < L::Q; q++) {
+            const float al = (float)L::t.w[q] * psin[q];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int c = L::t.c[q][d];
+                if (c == 1) F[d] += al;
+                else if (c == -1) F[d] -= al;
+            }
+        }
+        const float beta = -(float)a.eos.G * psic;
+#pragma unroll
+        for (int d = 0; d < 3; d++) Fi[d] = F[d] * beta;
+    }
+    float Ft[3], Fu[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const float grav = (rho - a.rho0) * (float)a.eos.g[d];
+        Ft[d] = Fi[d] + grav + (interior ? (float)a.eos.fext[d] : 0.f);
+        Fu[d] = a.fmodel == 0 ? Ft[d] : Fi[d] + grav;
+    }
+    float U[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        float j = 0.f;
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) {
+            const int c = L::t.c[q][d];
+            if (c == 1) j += v[q];
+            else if (c == -1) j -= v[q];
+        }
+        U[d] = (j + 0.5f * Fu[d]) / rho;
+    }
+    // MRT collision of f (liMRTCollision.c:33-106 / xuMRTCollision.c:42-92)
+    float m[L::Q], meq[L::Q], Sf[L::Q];
+    moments<L, float>(v, m);
+    const float umag = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
+    const float psi2 = psic * psic;
+    if (L::D == 2) {
+        meq[0] = rho;
+        meq[1] = rho * (-2.f + 3.f * umag);
+        meq[2] = rho * (1.f - 3.f * umag);
+        meq[3] = rho * U[0];
+        meq[4] = -rho * U[0];
+        meq[5] = rho * U[1];
+        meq[6] = -rho * U[1];
+        meq[7] = rho * (U[0] * U[0] - U[1] * U[1]);
+        meq[8] = rho * U[0] * U[1];
+        const float uF = U[0] * Fu[0] + U[1] * Fu[1];
+        const float fi2 = Fi[0] * Fi[0] + Fi[1] * Fi[1];
+        Sf[0] = 0.f;
+        Sf[1] = 6.f * uF + 12.f * a.sigma * fi2 / (psi2 * ((1.f / a.fl[1]) - 0.5f));
+        Sf[2] = -6.f * uF - 12.f * a.sigma * fi2 / (psi2 * ((1.f / a.fl[2]) - 0.5f));
+        Sf[3] = Fu[0];
+        Sf[4] = -Fu[0];
+        Sf[5] = Fu[1];
+        Sf[6] = -Fu[1];
+        Sf[7] = 2.f * (U[0] * Fu[0] - U[1] * Fu[1]);
+        Sf[8] = U[0] * Fu[1] + U[1] * Fu[0];
+    } else {
+        meq[0] = rho;
+        meq[1] = rho * (-1.f + umag);
+        meq[2] = rho * (1.f - 5.f * umag);
+        meq[3] = rho * U[0];
+        meq[4] = rho * (-7.f / 3.f) * U[0];
+        meq[5] = rho * U[1];
+        meq[6] = rho * (-7.f / 3.f) * U[1];
+        meq[7] = rho * U[2];
+        meq[8] = rho * (-7.f / 3.f) * U[2];
+        meq[9] = rho * (2.f * U[0] * U[0] - U[1] * U[1] - U[2] * U[2]);
+        meq[10] = rho * (U[1] * U[1] - U[2] * U[2]);
+        meq[11] = rho * U[0] * U[1];
+        meq[12] = rho * U[1] * U[2];
+        meq[13] = rho * U[0] * U[2];
+        meq[14] = 0.f;
+        const float uF = U[0] * Fu[0] + U[1] * Fu[1] + U[2] * Fu[2];
+        const float F2 = Fu[0] * Fu[0] + Fu[1] * Fu[1] + Fu[2] * Fu[2];
+        Sf[0] = 0.f;
+        Sf[1] = 2.f * uF + 6.f * a.sigma * F2 / (psi2 * ((1.f / a.fl[1]) - 0.5f));
+        Sf[2] = -10.f * uF;
+        Sf[3] = Fu[0];
+        Sf[4] = (-7.f / 3.f) * Fu[0];
+        Sf[5] = Fu[1];
+        Sf[6] = (-7.f / 3.f) * Fu[1];
+        Sf[7] = Fu[2];
+        Sf[8] = (-7.f / 3.f) * Fu[2];
+        Sf[9] = 4.f * U[0] * Fu[0] - 2.f * U[1] * Fu[1] - 2.f * U[2] * Fu[2];
+        Sf[10] = 2.f * U[1] * Fu[1] - 2.f * U[2] * Fu[2];
+        Sf[11] = U[0] * Fu[1] + U[1] * Fu[0];
+        Sf[12] = U[1] * Fu[2] + U[2] * Fu[1];
+        Sf[13] = U[0] * Fu[2] + U[2] * Fu[0];
+        Sf[14] = 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) m[q] = m[q] - a.fl[q] * (m[q] - meq[q]) + (1.f - 0.5f * a.fl[q]) * Sf[q];
+    if (a.surf) {
+        // liMRTSurfaceTension.c / xuMRTSurfaceTension.c (zero next to missing links)
+        bool full = true;
+#pragma unroll
+        for (int q = 1; q < L::Q; q++)
+            if (p.table && a.nb[q * N + id] == -1) full = false;
+        if (full) {
+            float Qt[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+            for (int q = 1; q < L::Q; q++) {
+                const float al = (float)L::t.w[q] * (psin[q] - psic);
+#pragma unroll
+                for (int r = 0; r < L::D; r++)
+#pragma unroll
+                    for (int s = r; s < L::D; s++) Qt[r][s] += al * (float)(L::t.c[q][r] * L::t.c[q][s]);
+            }
+            const float sc = 0.5f * psic * (float)a.eos.G * a.kappa;
+            if (L::D == 2) {
+                const float tr = (Qt[0][0] + Qt[1][1]) * sc;
+                m[1] += 1.5f * a.fl[1] * tr;
+                m[2] += -1.5f * a.fl[2] * tr;
+                m[7] += -a.fl[8] * (Qt[0][0] - Qt[1][1]) * sc;
+                m[8] += -a.fl[8] * Qt[0][1] * sc;
+            } else {
+                m[1] += 4.f * a.fl[1] * (Qt[0][0] + Qt[1][1] + Qt[2][2]) * sc / 5.f;
+                const float l10 = -a.fl[10] * sc;
+                m[9] += l10 * (2.f * Qt[0][0] - Qt[1][1] - Qt[2][2]);
+                m[10] += l10 * (Qt[1][1] - Qt[2][2]);
+                m[11] += l10 * Qt[0][1];
+                m[12] += l10 * Qt[1][2];
+                m[13] += l10 * Qt[0][2];
+            }
+        }
+    }
+    populations<L, float>(m, v);
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, v[q], a.fshift[q]);
+
+    if (THERMAL) {
+        pull_g<L, S>(a, p, v);
+        const float T = a.Tnew[id];
+        const float Sv = a.S[id];
+        float n[L::Q], neq[L::Q];
+        moments<L, float>(v, n);
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) neq[q] = 0.f;
+        neq[0] = T;
+        if (a.gmodel == 1) {
+            neq[1] = a.a1 * T;
+            neq[2] = a.a2 * T;
+        } else {
+            neq[1] = -2.f * T;
+            neq[2] = 2.f * T;
+        }
+        neq[3] = T * U[0];
+        neq[4] = -T * U[0];
+        neq[5] = T * U[1];
+        neq[6] = -T * U[1];
+        if (L::D == 3) {
+            neq[7] = T * U[2];
+            neq[8] = -T * U[2];
+        }
+        if (a.gmodel == 1) {
+            // myMRTCollision.c: n <- n - Q (n - neq) + (I - Q/2) Gamma
+            float d[L::Q];
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) d[q] = n[q] - neq[q];
+            float r[L::Q];
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) r[q] = a.gl[q] * d[q];
+            const int np = L::D == 2 ? 2 : 3;
+#pragma unroll
+            for (int pr = 0; pr < np; pr++) {
+                const int i0 = 3 + 2 * pr, i1 = 4 + 2 * pr;
+                r[i0] += a.gl[i1] * (a.gl[i0] / 2.f - 1.f) * d[i1];
+            }
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) n[q] -= r[q];
+            n[0] += (1.f - 0.5f * a.gl[0]) * Sv;
+        } else {
+            const float n4 = n[4], n6 = n[6];
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) n[q] = n[q] - a.gl[q] * (n[q] - neq[q]);
+            n[0] += Sv;
+            n[3] += (1.f - a.gl[3] / 2.f) * a.gl[4] * n4;
+            n[5] += (1.f - a.gl[5] / 2.f) * a.gl[6] * n6;
+        }
+        populations<L, float>(n, v);
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) st<S>(a.gout, q * a.P + id, v[q], a.gshift[q]);
+        a.Unew[id] = U[0];
+        a.Unew[N + id] = U[1];
+        a.Unew[2 * N + id] = U[2];
+    }
+    if (a.out_U) {
+        a.out_U[id] = U[0];
+        a.out_U[N + id] = U[1];
+        a.out_U[2 * N + id] = U[2];
+    }
+    if (a.out_rho) a.out_rho[id] = rho;
+    if (a.out_Fi) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            a.out_Fi[d * N + id] = Fi[d];
+            a.out_Ft[d * N + id] = Ft[d];
+        }
+    }
+}
+
+// Post-stream, post-boundary populations of the current time level (what the
+// reference holds in f.value after updateBoundaries), FP32 SoA.
+template <class L, typename S, bool G>
+__global__ void __launch_bounds__(256) k_fused_materialize(FArgs a, float* out) {
+    const NodePos p = node_pos<L>(a);
+    if (!p.valid) return;
+    float v[L::Q];
+    if (G) pull_g<L, S>(a, p, v);
+    else pull_f<L, S>(a, p, v);
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) out[q * a.N + p.id] = v[q];
+}
+
+// FP32 SoA -> storage (with shift)
+template <typename S>
+__global__ void k_to_storage(const float* __restrict__ in, void* out, int64_t N, int Q, FArgs a, int g) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= N * Q) return;
+    const int q = (int)(i / N);
+    st<S>(out, q * a.P + (i - q * N), in[i], g ? a.gshift[q] : a.fshift[q]);
+}
+
+struct Fused {
+    int lb = 0;
+    bool thermal = false;
+    bool half = false;
+    void* fa = nullptr;
+    void* fb = nullptr;
+    void* ga = nullptr;
+    void* gb = nullptr;
+    float* rho = nullptr;
+    float* sumg = nullptr;
+    float* T[2] = {nullptr, nullptr};
+    float* U[2] = {nullptr, nullptr};
+    int cur = 0;                   // T[cur], U[cur] hold the current time level
+    uint8_t* bcf = nullptr;
+    uint8_t* bcg = nullptr;
+    float* bcTw = nullptr;
+    float* bcPert = nullptr;
+    int* rnd_off = nullptr;
+    std::vector<std::pair<int, int>> frdt_lists;   // (field, list) of fixedRandomT
+    bool need_prologue = true;     // post-collision buffers must be rebuilt from the generic state
+    bool generic_current = true;   // generic device arrays describe the current time level
+    int last_draw = 0;             // fixedRandomT drawing flag of the last step
+    int64_t P = 0;                 // population plane pitch
+    int face_table = 0;            // faces whose links differ from the periodic wrap
+    float fshift[15] = {0}, gshift[15] = {0};
+};
+
+Fused* F(mplb_case* c) { return (Fused*)c->fz; }
+
+#ifndef MPLB_PITCH_PAD
+#define MPLB_PITCH_PAD 0
+#endif
+// Population planes are MPLB_PITCH_PAD elements longer than the lattice so that the
+// Q concurrent streams of a pull do not sit at a power-of-two distance apart.
+int64_t pitch_for(int64_t N) { return ((N + 63) / 64) * 64 + MPLB_PITCH_PAD; }
+
+int err(mplb_case* c, int code, const std::string& m) {
+    c->err = m;
+    return code;
+}
+
+#define FCHK(c, call)                                                                            \
+    do {                                                                                         \
+        hipError_t _e = (call);                                                                  \
+        if (_e != hipSuccess) return err((c), MPLB_E_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+int reverse_of(const mplb_case* c, int k) { return c->mesh.lbmodel == 0 ? D2Q9::t.rev[k] : D3Q15::t.rev[k]; }
+
+FArgs make_args(mplb_case* c) {
+    Fused* z = F(c);
+    FArgs a;
+    std::memset(&a, 0, sizeof(a));
+    const Mesh& m = c->mesh;
+    a.nx = m.nx;
+    a.ny = m.ny;
+    a.nz = m.nz;
+    a.N = m.N;
+    a.P = z->P;
+    a.nxy = (int64_t)m.nx * m.ny;
+    a.nblocks = (int)((m.N + 255) / 256);
+    a.face_table = z->face_table;
+    a.nb = c->nb;
+    a.onbnd = c->onbnd;
+    a.bcf = z->bcf;
+    a.bcg = z->bcg;
+    a.bcTw = z->bcTw;
+    a.bcPert = z->bcPert;
+    a.rnd_off = z->rnd_off;
+    a.rnd = c->rnd;
+    a.eos = c->eos;
+    a.rho0 = (float)c->rho0;
+    const ModelParams& fp = c->fc[0].mp;
+    a.fmodel = fp.model == M_LIMRT ? 0 : 1;
+    a.surf = fp.surf;
+    a.sigma = (float)fp.sigma;
+    a.kappa = (float)fp.kappa;
+    for (int q = 0; q < 15; q++) a.fl[q] = (float)fp.Lambda[q];
+    if (c->fc[1].present) {
+        const ModelParams& gp = c->fc[1].mp;
+        a.gmodel = gp.model == M_MYMRT ? 1 : 3;
+        a.a1 = (float)gp.alpha1;
+        a.a2 = (float)gp.alpha2;
+        for (int q = 0; q < 15; q++) a.gl[q] = (float)gp.Lambda[q];
+    }
+    a.psi = (float*)c->psi;
+    a.S = (float*)c->ss;
+    a.rho = z->rho;
+    a.sumg = z->sumg;
+    a.Told = z->thermal ? z->T[z->cur] : (const float*)c->temp;
+    a.Tnew = z->thermal ? z->T[1 - z->cur] : nullptr;
+    a.Uold = z->thermal ? z->U[z->cur] : (const float*)c->U;
+    a.Unew = z->thermal ? z->U[1 - z->cur] : nullptr;
+    a.Tuniform = 0;
+    std::memcpy(a.fshift, z->fshift, sizeof(a.fshift));
+    std::memcpy(a.gshift, z->gshift, sizeof(a.gshift));
+    return a;
+}
+
+template <class L, typename S>
+void launch_step(mplb_case* c, FArgs a, bool thermal, bool outputs) {
+    const dim3 grid((unsigned)a.nblocks), block(256);
+    if (outputs) {
+        a.out_rho = (float*)c->rho;
+        a.out_Fi = (float*)c->Fi;
+        a.out_Ft = (float*)c->Ft;
+        if (!thermal) a.out_U = (float*)c->U;
+    }
+    if (thermal) {
+        k_fused_density<L, S, true><<<grid, block, 0, c->stream>>>(a);
+        k_fused_heat<L><<<grid, block, 0, c->stream>>>(a);
+        k_fused_collide<L, S, true><<<grid, block, 0, c->stream>>>(a);
+    } else {
+        k_fused_density<L, S, false><<<grid, block, 0, c->stream>>>(a);
+        k_fused_collide<L, S, false><<<grid, block, 0, c->stream>>>(a);
+    }
+}
+
+// Time each kernel separately when the case is timing.
+struct Ev {
+    mplb_case* c;
+    int idx;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Ev(mplb_case* cc, int i) : c(cc), idx(i) {
+        if (!c->timing) return;
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipEventRecord(e0, c->stream);
+    }
+    ~Ev() {
+        if (!c->timing) return;
+        hipEventRecord(e1, c->stream);
+        c->ev_pool.push_back(e0);
+        c->ev_pool.push_back(e1);
+        c->ev_pool.push_back((hipEvent_t)(intptr_t)idx);
+    }
+};
+
+template <class L, typename S>
+void launch_step_timed(mplb_case* c, FArgs a, bool thermal, bool outputs) {
+    const dim3 grid((unsigned)a.nblocks), block(256);
+    if (outputs) {
+        a.out_rho = (float*)c->rho;
+        a.out_Fi = (float*)c->Fi;
+        a.out_Ft = (float*)c->Ft;
+        if (!thermal) a.out_U = (float*)c->U;
+    }
+    // timer slots: 6 = fused collide (dominant), 7 = density/heat passes
+    if (thermal) {
+        {
+            Ev e(c, 7);
+            k_fused_density<L, S, true><<<grid, block, 0, c->stream>>>(a);
+        }
+        {
+            Ev e(c, 5);
+            k_fused_heat<L><<<grid, block, 0, c->stream>>>(a);
+        }
+        Ev e(c, 6);
+        k_fused_collide<L, S, true><<<grid, block, 0, c->stream>>>(a);
+    } else {
+        {
+            Ev e(c, 7);
+            k_fused_density<L, S, false><<<grid, block, 0, c->stream>>>(a);
+        }
+        Ev e(c, 6);
+        k_fused_collide<L, S, false><<<grid, block, 0, c->stream>>>(a);
+    }
+}
+
+template <class L, typename S>
+int run_prologue(mplb_case* c) {
+    // generic state (post-stream populations and macroscopic fields of the current time
+    // level) -> post-collision buffers of the next step
+    Fused* z = F(c);
+    DevState<float> st;
+    st.N = c->mesh.N;
+    st.nb = c->nb;
+    st.onbnd = c->onbnd;
+    st.rho = (float*)c->rho;
+    st.temp = (float*)c->temp;
+    st.p = (float*)c->p;
+    st.U = (float*)c->U;
+    st.Fi = (float*)c->Fi;
+    st.Ft = (float*)c->Ft;
+    st.ss = (float*)c->ss;
+    st.psi = (float*)c->psi;
+    GenericOps<float>::psi(c->mesh.lbmodel, c->stream, c->eos, st);
+    const int64_t N = c->mesh.N;
+    const int Q = c->mesh.Q;
+    const size_t bytes = sizeof(float) * Q * N;
+    // collide copies of f and g (the generic arrays keep the current level)
+    float* tmp = nullptr;
+    FCHK(c, hipMalloc(&tmp, bytes));
+    FArgs a = make_args(c);
+    for (int fld = 0; fld < (z->thermal ? 2 : 1); fld++) {
+        FCHK(c, hipMemcpyAsync(tmp, c->f[fld], bytes, hipMemcpyDeviceToDevice, c->stream));
+        GenericOps<float>::collide(c->mesh.lbmodel, c->stream, c->eos, c->fc[fld].mp, st, tmp, c->rho0);
+        void* dst = fld == 0 ? z->fa : z->ga;
+        k_to_storage<S><<<(unsigned)((N * Q + 255) / 256), 256, 0, c->stream>>>(tmp, dst, N, Q, a, fld);
+    }
+    if (z->thermal) {
+        FCHK(c, hipMemcpyAsync(z->T[z->cur], c->temp, sizeof(float) * N, hipMemcpyDeviceToDevice, c->stream));
+        FCHK(c, hipMemcpyAsync(z->U[z->cur], c->U, sizeof(float) * 3 * N, hipMemcpyDeviceToDevice, c->stream));
+    }
+    FCHK(c, hipStreamSynchronize(c->stream));
+    hipFree(tmp);
+    z->need_prologue = false;
+    z->generic_current = true;
+    return 0;
+}
+
+template <class L, typename S>
+int do_step(mplb_case* c) {
+    Fused* z = F(c);
+    if (z->need_prologue) {
+        int rc = run_prologue<L, S>(c);
+        if (rc) return rc;
+    }
+    FArgs a = make_args(c);
+    a.fin = z->fa;
+    a.fout = z->fb;
+    a.gin = z->ga;
+    a.gout = z->gb;
+    a.first_call = c->current == 0 ? 1 : 0;
+    // fixedRandomT: draws while localTStep < psteps (one flag for all such lists)
+    a.draw = 0;
+    for (auto& fl : z->frdt_lists) {
+        FieldCfg& fc = c->fc[fl.first];
+        if (fc.localT[fl.second] < fc.psteps[fl.second]) a.draw = 1;
+        fc.localT[fl.second]++;
+    }
+    z->last_draw = a.draw;
+    if (c->timing) launch_step_timed<L, S>(c, a, z->thermal, false);
+    else launch_step<L, S>(c, a, z->thermal, false);
+    std::swap(z->fa, z->fb);
+    std::swap(z->ga, z->gb);
+    if (z->thermal) z->cur = 1 - z->cur;
+    z->generic_current = false;
+    return 0;
+}
+
+// After do_step the buffers hold: fa = post-collision of step n+1, fb = post-collision
+// of step n (input of the last K2).  Materialising level n pulls from fb.
+template <class L, typename S>
+int materialize(mplb_case* c) {
+    Fused* z = F(c);
+    if (z->generic_current) return 0;
+    FArgs a = make_args(c);
+    const dim3 grid((unsigned)a.nblocks), block(256);
+    // the last step's inputs: post-collision of level n in fb/gb, old T/U = previous
+    a.fin = z->fb;
+    a.gin = z->gb;
+    a.draw = z->last_draw;
+    if (z->thermal) {
+        a.Told = z->T[1 - z->cur];
+        a.Uold = z->U[1 - z->cur];
+    }
+    k_fused_materialize<L, S, false><<<grid, block, 0, c->stream>>>(a, (float*)c->f[0]);
+    if (z->thermal) k_fused_materialize<L, S, true><<<grid, block, 0, c->stream>>>(a, (float*)c->f[1]);
+    // macroscopic outputs: re-run the collide pass (deterministic, rewrites fa)
+    a.fout = z->fa;
+    a.gout = z->ga;
+    a.first_call = 0;
+    if (z->thermal) {
+        a.Tnew = z->T[z->cur];
+        a.Unew = z->U[z->cur];
+    }
+    a.out_rho = (float*)c->rho;
+    a.out_Fi = (float*)c->Fi;
+    a.out_Ft = (float*)c->Ft;
+    a.out_U = (float*)c->U;
+    if (z->thermal) k_fused_collide<L, S, true><<<grid, block, 0, c->stream>>>(a);
+    else k_fused_collide<L, S, false><<<grid, block, 0, c->stream>>>(a);
+    if (z->thermal) {
+        const int64_t N = c->mesh.N;
+        FCHK(c, hipMemcpyAsync(c->temp, z->T[z->cur], sizeof(float) * N, hipMemcpyDeviceToDevice, c->stream));
+    }
+    FCHK(c, hipStreamSynchronize(c->stream));
+    z->generic_current = true;
+    return 0;
+}
+
+template <typename Fn>
+int dispatch(mplb_case* c, Fn fn) {
+    Fused* z = F(c);
+    if (z->lb == 0) {
+        if (z->half) return fn(D2Q9{}, _Float16{});
+        return fn(D2Q9{}, float{});
+    }
+    if (z->half) return fn(D3Q15{}, _Float16{});
+    return fn(D3Q15{}, float{});
+}
+
+}  // namespace
+
+bool fused_supported(const mplb_case* c, std::string* why) {
+    auto no = [&](const char* m) {
+        if (why) *why = m;
+        return false;
+    };
+    if (c->opts.precision == MPLB_FP64) return no("the fused kernels compute in FP32");
+    const Mesh& m = c->mesh;
+    if (m.nz > 1 && m.d == 2) return no("2D box with nz > 1");
+    const int fm = c->fc[0].mp.model;
+    if (!(fm == M_LIMRT && m.lbmodel == 0) && !(fm == M_XUMRT && m.lbmodel == 1)) return no("f model/lattice");
+    if (c->update[0] == 0) return no("frozen f field");
+    const bool thermal = c->fc[1].present && c->update[1];
+    if (thermal) {
+        const int gm = c->fc[1].mp.model;
+        if (gm != M_MYMRT && !(gm == M_LITEMPMRT && m.lbmodel == 0)) return no("g model");
+    }
+    // every node on at most one boundary list per field, only node-local or
+    // interior-neighbour conditions, and no conditional on a node another condition writes
+    std::vector<uint8_t> seen(m.N, 0);
+    for (size_t b = 0; b < m.bd.size(); b++) {
+        for (int f = 0; f < 2; f++) {
+            if (f == 1 && !thermal) continue;
+            const int t = c->fc[f].bctype[b];
+            if (t == BD_PERIODIC) continue;
+            if (f == 0 && t != BD_BBACK) return no("f boundary type other than bounceBack/periodic");
+            if (f == 1 && t != BD_BBACK && t != BD_FT && t != BD_FRDT) return no("g boundary type");
+            for (int32_t id : m.bd[b]) {
+                if (seen[id] & (1 << f)) return no("node on two boundary lists");
+                seen[id] |= (uint8_t)(1 << f);
+            }
+        }
+    }
+    if (thermal) {
+        for (size_t b = 0; b < m.bd.size(); b++) {
+            const int t = c->fc[1].bctype[b];
+            if (t != BD_FT && t != BD_FRDT) continue;
+            for (int32_t id : m.bd[b])
+                for (int k = 0; k < m.Q; k++) {
+                    if (m.nb[(int64_t)id * m.Q + k] != -1) continue;
+                    const int32_t nbid = m.nb[(int64_t)id * m.Q + reverse_of(c, k)];
+                    if (nbid != -1 && (seen[nbid] & 2) && m.nb[(int64_t)nbid * m.Q + k] == -1)
+                        return no("fixedT reads a population another condition writes");
+                }
+        }
+    }
+    return true;
+}
+
+int fused_create(mplb_case* c) {
+    Fused* z = new Fused();
+    c->fz = z;
+    const Mesh& m = c->mesh;
+    const int64_t N = m.N;
+    const int Q = m.Q;
+    z->lb = m.lbmodel;
+    z->thermal = c->fc[1].present && c->update[1];
+    z->half = c->opts.precision == MPLB_FP16S;
+    const size_t sb = z->half ? 2 : 4;
+    z->P = pitch_for(N);
+    FCHK(c, hipMalloc(&z->fa, sb * Q * z->P));
+    FCHK(c, hipMalloc(&z->fb, sb * Q * z->P));
+    if (z->thermal) {
+        FCHK(c, hipMalloc(&z->ga, sb * Q * z->P));
+        FCHK(c, hipMalloc(&z->gb, sb * Q * z->P));
+        FCHK(c, hipMalloc(&z->sumg, sizeof(float) * N));
+        for (int i = 0; i < 2; i++) {
+            FCHK(c, hipMalloc(&z->T[i], sizeof(float) * N));
+            FCHK(c, hipMalloc(&z->U[i], sizeof(float) * 3 * N));
+        }
+    }
+    FCHK(c, hipMalloc(&z->rho, sizeof(float) * N));
+    // per-node boundary codes
+    std::vector<uint8_t> bcf(N, 0), bcg(N, 0);
+    std::vector<float> tw(N, 0.f), pert(N, 0.f);
+    std::vector<int> off(N, 0);
+    bool any_frdt = false;
+    for (size_t b = 0; b < m.bd.size(); b++) {
+        for (int f = 0; f < 2; f++) {
+            if (f == 1 && !z->thermal) continue;
+            const FieldCfg& fc = c->fc[f];
+            const int t = fc.bctype[b];
+            uint8_t code = t == BD_BBACK ? FB_BBACK : t == BD_FT ? FB_FT : t == BD_FRDT ? FB_FRDT : FB_NONE;
+            if (code == FB_NONE) continue;
+            int cnt = 0;
+            for (int32_t id : m.bd[b]) {
+                (f == 0 ? bcf : bcg)[id] = code;
+                if (f == 1) {
+                    tw[id] = (float)fc.bcval[b][0];
+                    pert[id] = (float)fc.pert[b];
+                    if (code == FB_FRDT) {
+                        off[id] = cnt;
+                        for (int k = 0; k < Q; k++)
+                            if (m.nb[(int64_t)id * Q + k] == -1 && m.nb[(int64_t)id * Q + reverse_of(c, k)] != -1) cnt++;
+                    }
+                }
+            }
+            if (code == FB_FRDT) {
+                any_frdt = true;
+                z->frdt_lists.push_back({f, (int)b});
+            }
+        }
+    }
+    // Faces whose links are not the duplicate-node periodic wrap, or that carry a
+    // boundary condition, are read through the neighbour table.
+    {
+        const int nx = m.nx, ny = m.ny, nz = m.nz;
+        const int64_t nxy = (int64_t)nx * ny;
+        auto wrapc = [](int u, int n) { return u < 0 ? u + (n - 1) : (u > n - 1 ? u - (n - 1) : u); };
+        int mask = 0;
+        for (int64_t id = 0; id < N; id++) {
+            const int i = (int)(id % nx), j = (int)((id / nx) % ny), k = (int)(id / nxy);
+            int faces = (i == 0 ? 1 : 0) | (i == nx - 1 ? 2 : 0) | (j == 0 ? 4 : 0) | (j == ny - 1 ? 8 : 0);
+            if (m.d == 3) faces |= (k == 0 ? 16 : 0) | (k == nz - 1 ? 32 : 0);
+            if (!faces || (faces & ~mask) == 0) continue;
+            bool differs = bcf[id] != FB_NONE || bcg[id] != FB_NONE;
+            for (int q = 0; q < Q && !differs; q++) {
+                const int* cq = m.lbmodel == 0 ? D2Q9::t.c[q] : D3Q15::t.c[q];
+                const int64_t want = wrapc(i - cq[0], nx) + (int64_t)wrapc(j - cq[1], ny) * nx +
+                                     (m.d == 3 ? (int64_t)wrapc(k - cq[2], nz) * nxy : 0);
+                if (m.nb[id * Q + q] != want) differs = true;
+            }
+            if (differs) mask |= faces;
+        }
+        z->face_table = mask;
+    }
+    FCHK(c, hipMalloc(&z->bcf, N));
+    FCHK(c, hipMemcpy(z->bcf, bcf.data(), N, hipMemcpyHostToDevice));
+    FCHK(c, hipMalloc(&z->bcg, N));
+    FCHK(c, hipMemcpy(z->bcg, bcg.data(), N, hipMemcpyHostToDevice));
+    FCHK(c, hipMalloc(&z->bcTw, sizeof(float) * N));
+    FCHK(c, hipMemcpy(z->bcTw, tw.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    FCHK(c, hipMalloc(&z->bcPert, sizeof(float) * N));
+    FCHK(c, hipMemcpy(z->bcPert, pert.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    FCHK(c, hipMalloc(&z->rnd_off, sizeof(int) * N));
+    FCHK(c, hipMemcpy(z->rnd_off, off.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    (void)any_frdt;
+    return 0;
+}
+
+void fused_destroy(mplb_case* c) {
+    Fused* z = F(c);
+    if (!z) return;
+    void* ptrs[] = {z->fa, z->fb, z->ga, z->gb, z->rho, z->sumg, z->T[0], z->T[1], z->U[0], z->U[1],
+                    z->bcf, z->bcg, z->bcTw, z->bcPert, z->rnd_off};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    delete z;
+    c->fz = nullptr;
+}
+
+static void set_shifts(mplb_case* c) {
+    Fused* z = F(c);
+    if (!z->half) return;
+    // deviation storage: f_k - w_k rho_ref, g_k - geq_k(T_ref, 0); reference values are
+    // the lattice means of the initial fields
+    const int64_t N = c->mesh.N;
+    std::vector<float> h(N);
+    hipMemcpy(h.data(), c->rho, sizeof(float) * N, hipMemcpyDeviceToHost);
+    double rs = 0;
+    for (float v : h) rs += v;
+    const double rho_ref = rs / (double)N;
+    double ts = 0;
+    hipMemcpy(h.data(), c->temp, sizeof(float) * N, hipMemcpyDeviceToHost);
+    for (float v : h) ts += v;
+    const double T_ref = ts / (double)N;
+    const int Q = c->mesh.Q;
+    double geq[15] = {0};
+    const double U0[3] = {0, 0, 0};
+    if (c->fc[1].present) {
+        const ModelParams& gp = c->fc[1].mp;
+        if (c->mesh.lbmodel == 0) {
+            if (gp.model == M_MYMRT) my_geq<D2Q9, double>(T_ref, U0, gp.alpha1, gp.alpha2, geq);
+            else litemp_geq<D2Q9, double>(T_ref, U0, geq);
+        } else {
+            my_geq<D3Q15, double>(T_ref, U0, gp.alpha1, gp.alpha2, geq);
+        }
+    }
+    for (int q = 0; q < Q; q++) {
+        const double w = c->mesh.lbmodel == 0 ? D2Q9::t.omega[q] : D3Q15::t.omega[q];
+        z->fshift[q] = (float)(w * rho_ref);
+        z->gshift[q] = (float)geq[q];
+    }
+}
+
+int fused_after_init(mplb_case* c) {
+    set_shifts(c);
+    F(c)->need_prologue = true;
+    F(c)->generic_current = true;
+    return dispatch(c, [&](auto L, auto S) { return run_prologue<decltype(L), decltype(S)>(c); });
+}
+
+int fused_step(mplb_case* c) {
+    return dispatch(c, [&](auto L, auto S) { return do_step<decltype(L), decltype(S)>(c); });
+}
+
+int fused_sync_to_generic(mplb_case* c) {
+    if (!c->fz) return 0;
+    return dispatch(c, [&](auto L, auto S) { return materialize<decltype(L), decltype(S)>(c); });
+}
+
+int fused_after_write(mplb_case* c) {
+    // rho_0 may have changed: the next step's collision is recomputed from the current
+    // level on the next fused step
+    F(c)->need_prologue = true;
+    return 0;
+}
+
+int fused_set_field(mplb_case* c, const char*, const double*, size_t) {
+    int rc = fused_sync_to_generic(c);
+    if (rc) return rc;
+    F(c)->need_prologue = true;
+    return 1;
+}
+
+int fused_get_field(mplb_case* c, const char*, double*, size_t) {
+    int rc = fused_sync_to_generic(c);
+    if (rc) return rc;
+    return 1;
+}
 
 }  // namespace mplb

@@ -1,0 +1,123 @@
+"""Initial fields for benchmark and parity cases.
+
+Replaces the reference's setInitialFields generators for the synthetic workloads:
+the reference's random field is seeded from the clock (randomScalarField.c:40) and has
+no Maxwell construction, so runs could not be repeated.  Here:
+  * maxwell()      coexistence densities of an EOS at temperature T from the equal
+                   pressure and equal area (in specific volume) conditions, FP64;
+  * noise()        uniform +-amp noise from splitmix64 (seed), drawn on the periodic
+                   sites and copied onto the duplicate end planes so the two stored
+                   copies of a site start identical;
+  * sphere()/pool() the reference's tanh profile
+                   0.5 (in + out) + 0.5 (out - in) tanh(2 (r - R) / W)
+                   (sphereBoundedScalarField.c:184) around a centre or along the
+                   vertical axis.
+"""
+import numpy as np
+from scipy import optimize
+
+
+def p_eos(rho, T, eos="vdw", a=0.5, b=4.0, R=1.0, M=1.0, w=0.344):
+    """Same forms as EOS/p_eos.c:7-64."""
+    rho = np.asarray(rho, float)
+    if eos == "vdw":
+        return rho * R * T / (M - rho * b) - a * rho * rho / (M * M)
+    if eos == "cs":
+        al = b * rho / 4.0
+        be = 1 - al
+        return rho * R * T * ((1 + al + al * al - al ** 3) / be ** 3) - a * rho * rho
+    if eos == "pr":
+        tc = 0.170151343 * a / (b * R)
+        phi = 1 + (0.37464 + 1.5422 * w - 0.26992 * w * w) * (1 - np.sqrt(T / tc))
+        return rho * R * T / (1 - b * rho) - a * phi * phi * rho * rho / (1 + 2 * b * rho - (b * rho) ** 2)
+    if eos == "ideal":
+        return rho * R * T
+    raise ValueError(eos)
+
+
+def vdw_critical(a=0.5, b=4.0, R=1.0, M=1.0):
+    """rho_c = M / (3 b), T_c = 8 a / (27 b R) (with the reference's M scaling)."""
+    return M / (3.0 * b), 8.0 * a / (27.0 * b * R * M)
+
+
+def maxwell(T, eos="vdw", **kw):
+    """(rho_vapour, rho_liquid) at temperature T below the critical point."""
+    if eos != "vdw":
+        raise NotImplementedError("Maxwell construction implemented for van der Waals")
+    a, b, R, M = kw.get("a", 0.5), kw.get("b", 4.0), kw.get("R", 1.0), kw.get("M", 1.0)
+    rc, tc = vdw_critical(a, b, R, M)
+    if not 0 < T < tc:
+        raise ValueError("T must be below T_c = %g" % tc)
+    p = lambda r: p_eos(r, T, "vdw", a=a, b=b, R=R, M=M)
+    dp = lambda r: R * T * M / (M - r * b) ** 2 - 2 * a * r / (M * M)
+    rmax = (M / b) * (1 - 1e-13)
+    lo = optimize.brentq(dp, 1e-12 * rc, rc)      # vapour-side spinodal
+    hi = optimize.brentq(dp, rc, rmax)            # liquid-side spinodal
+
+    def roots(ps):
+        rv = optimize.brentq(lambda r: p(r) - ps, 0.0, lo, xtol=1e-300, rtol=1e-15)
+        rl = optimize.brentq(lambda r: p(r) - ps, hi, rmax, xtol=1e-16, rtol=1e-15)
+        return rv, rl
+
+    def prim(r, ps):
+        # antiderivative of (p(r) - ps) / r^2 for the van der Waals p
+        return R * T / M * np.log(r / (M - b * r)) - a * r / (M * M) + ps / r
+
+    def area(ps):
+        # equal areas in specific volume v = 1/rho: integral of (p - ps) dv = 0
+        rv, rl = roots(ps)
+        return prim(rl, ps) - prim(rv, ps)
+
+    pmin = max(p(hi) * (1 + 1e-12), p(lo) * 1e-10)
+    pmax = p(lo) * (1 - 1e-12)
+    ps = optimize.brentq(area, pmin, pmax, xtol=1e-18, rtol=1e-15)
+    rv, rl = roots(ps)
+    return float(rv), float(rl)
+
+
+def splitmix64(seed, n):
+    """n uniform doubles in [0, 1) from the splitmix64 sequence started at seed."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def noise(shape, amp, seed, periodic=(True, True, True)):
+    """+-amp noise on a (nz, ny, nx) lattice; periodic axes copy plane 0 onto the
+    duplicate last plane."""
+    nz, ny, nx = shape
+    uz = nz - 1 if (periodic[2] and nz > 1) else nz
+    uy = ny - 1 if periodic[1] else ny
+    ux = nx - 1 if periodic[0] else nx
+    core = amp * (2.0 * splitmix64(seed, uz * uy * ux) - 1.0)
+    core = core.reshape(uz, uy, ux)
+    out = np.empty(shape)
+    out[:uz, :uy, :ux] = core
+    if ux < nx:
+        out[:uz, :uy, ux] = core[:, :, 0]
+    if uy < ny:
+        out[:uz, uy, :] = out[:uz, 0, :]
+    if uz < nz:
+        out[uz, :, :] = out[0, :, :]
+    return out
+
+
+def sphere(shape, centre, radius, inside, outside, width):
+    nz, ny, nx = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r = np.sqrt((i - centre[0]) ** 2 + (j - centre[1]) ** 2 + (k - centre[2]) ** 2)
+    return 0.5 * (inside + outside) + 0.5 * (outside - inside) * np.tanh(2.0 * (r - radius) / width)
+
+
+def pool(shape, height, below, above, width, axis=0):
+    """Layer of density `below` under `height` along axis 0 (z) of (nz, ny, nx)."""
+    nz, ny, nx = shape
+    h = np.arange(shape[axis]).astype(float)
+    prof = 0.5 * (below + above) + 0.5 * (above - below) * np.tanh(2.0 * (h - height) / width)
+    s = [1, 1, 1]
+    s[axis] = shape[axis]
+    return np.broadcast_to(prof.reshape(s), shape).copy()
