@@ -166,6 +166,15 @@ def main():
         per_unit = 2 * 15 * s_pdf + s_mac                         # f in/out; psi in
     achieved = per_unit * N / (k_ms / 1e3) / 1e9 if k_ms > 0 else None
     step_gbs = mlups * 1e6 * blu / 1e9
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3 counter
+    # runs of this workload (profiles/fused_traffic.json), when one exists for it
+    traffic = None
+    key = "d3q15_%d_%s_%s" % (args.size, args.precision, "thermal" if args.thermal else "iso")
+    tpath = os.path.join(ROOT, "profiles", "fused_traffic.json")
+    if kname == "fused" and os.path.exists(tpath):
+        rec = json.load(open(tpath)).get("kernels", {}).get(key)
+        if rec:
+            traffic = rec["hbm_bytes_per_launch"]
     out = {
         "metric": "D3Q15 pseudopot-MRT MLUPS",
         "value": round(mlups, 2),
@@ -188,7 +197,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": kname, "kernel_ms": round(k_ms, 5),
+                     "traffic": traffic, "kernel": kname, "kernel_ms": round(k_ms, 5),
                      "bytes_per_unit": per_unit, "units_per_launch": N},
         "kernels_ms": {k: round(v[0], 5) for k, v in r["kern"].items()},
         "finite": r["finite"],

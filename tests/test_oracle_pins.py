@@ -1,0 +1,228 @@
+"""Pins of the CPU oracle that need no reference run: closed forms, conservation laws
+and fixed points, chosen so that a plausible restatement mistake (a dropped term, a
+wrong sign or index, a transposed table) fails one of them."""
+import numpy as np
+import pytest
+
+import _oracle as O
+from _cases import oracle_params
+from multiphaselb_amd.casefiles import CaseSpec
+from multiphaselb_amd.initfields import maxwell, p_eos, vdw_critical
+
+LAM9 = (1.0, 1.1, 1.1, 1.0, 1.1, 1.0, 1.1, 0.8, 0.8)
+LAM15 = (1.0, 1.1, 1.1, 1.0, 1.1, 1.0, 1.1, 1.0, 1.1, 1.0, 1.0, 1.0, 1.0, 1.0, 1.2)
+GLAM9 = (0.0, 1.0, 1.0, 1.2, 1.0, 1.2, 1.0, 1.0, 1.0)
+GLAM15 = (0.0,) + (1.0,) * 2 + (1.2, 1.0) * 3 + (1.0,) * 6
+
+
+@pytest.mark.parametrize("model", [O.D2Q9, O.D3Q15])
+def test_lattice_isotropy_and_inverse(model):
+    t = O.lattice_tables(model)
+    c, w, om = t["vel"].astype(float), t["w"], t["omega"]
+    assert abs(om.sum() - 1.0) < 1e-15
+    d = t["d"]
+    second = np.einsum("k,ki,kj->ij", om, c, c)[:d, :d]
+    assert np.allclose(second, t["cs2"] * np.eye(d), atol=1e-15)
+    fourth = np.einsum("k,ki,kj,kl,km->ijlm", om, c, c, c, c)[:d, :d, :d, :d]
+    I = np.eye(d)
+    iso = t["cs2"] ** 2 * (np.einsum("ij,lm->ijlm", I, I) + np.einsum("il,jm->ijlm", I, I) +
+                           np.einsum("im,jl->ijlm", I, I))
+    if model == O.D2Q9:
+        assert np.allclose(fourth, iso, atol=1e-15)
+    else:  # D3Q15 is isotropic to fourth order only for the diagonal terms it needs
+        assert np.allclose(np.einsum("iijj->ij", fourth), np.einsum("iijj->ij", iso), atol=1e-15)
+    assert np.allclose(np.einsum("k,ki,kj->ij", w, c, c)[:d, :d], np.eye(d), atol=1e-15)
+    assert np.abs(t["M"] @ t["invM"] - np.eye(t["Q"])).max() < 1e-15
+    for k in range(t["Q"]):
+        assert np.array_equal(c[t["rev"][k]], -c[k])
+    # rows of M are the moments the collision relaxes: density, momentum
+    assert np.array_equal(t["M"][0], np.ones(t["Q"]))
+    assert np.array_equal(t["M"][3], c[:, 0])
+
+
+def test_eos_closed_forms():
+    p = O.make_params(eos=O.VDW, a=0.5, b=4.0, R=1.0, M=1.0)
+    rc, tc = vdw_critical(0.5, 4.0)
+    L = O.lib()
+    pe = lambda r, T: L.orc_p_eos(p, r, T)
+    h = 1e-4 * rc
+    d1 = (pe(rc + h, tc) - pe(rc - h, tc)) / (2 * h)
+    d2 = (pe(rc + h, tc) - 2 * pe(rc, tc) + pe(rc - h, tc)) / h ** 2
+    assert abs(d1) < 1e-7 and abs(d2) < 1e-3           # inflection point at (rho_c, T_c)
+    assert pe(rc, tc) == pytest.approx(0.5 / (27 * 16.0), rel=1e-12)   # p_c = a / (27 b^2)
+    q = O.make_params(eos=O.IDEAL, R=0.7)
+    assert L.orc_p_eos(q, 0.3, 0.2) == 0.3 * 0.7 * 0.2
+    cs = O.make_params(eos=O.CARSTAR, a=0.3, b=0.2, R=1.0)
+    r = 1e-6
+    assert L.orc_p_eos(cs, r, 0.5) == pytest.approx(r * 0.5, rel=1e-6)  # dilute limit
+    # the oracle and the host initialiser agree on every EOS
+    for eos, name in ((O.VDW, "vdw"), (O.CARSTAR, "cs"), (O.PENGROB, "pr"), (O.IDEAL, "ideal")):
+        pp = O.make_params(eos=eos, a=0.3, b=0.5, w_pr=0.3, R=1.0)
+        for rho, T in ((0.1, 0.05), (0.6, 0.04)):
+            assert L.orc_p_eos(pp, rho, T) == pytest.approx(float(p_eos(rho, T, name, a=0.3, b=0.5, w=0.3)), rel=1e-12)
+
+
+def test_maxwell_construction_invariants():
+    rc, tc = vdw_critical()
+    for tr in (0.6, 0.8, 0.9, 0.95):
+        T = tr * tc
+        rv, rl = maxwell(T)
+        ps = float(p_eos(rl, T))
+        assert float(p_eos(rv, T)) == pytest.approx(ps, rel=1e-10)
+        v = np.linspace(1.0 / rl, 1.0 / rv, 400001)          # equal areas in specific volume
+        integrand = p_eos(1.0 / v, T) - ps
+        area = np.trapezoid(integrand, v)
+        assert abs(area) < 1e-7 * np.trapezoid(np.abs(integrand), v)
+    rv, rl = maxwell(0.9 * tc)
+    assert (rv / rc, rl / rc) == pytest.approx((0.4257, 1.6573), abs=2e-4)   # tabulated reduced VdW values
+
+
+def _random_case(model, f_model, nx=6, ny=6, nz=6, seed=0, g=(0.0, 0.0, 0.0)):
+    spec = CaseSpec(lbmodel=model, nx=nx, ny=ny, nz=nz if model else 1, btype=3 if model == 0 else 5,
+                    f_model=f_model, f_Lambda=LAM9 if model == 0 else LAM15, f_sigma=0.1,
+                    g_model=1, g_Lambda=GLAM9 if model == 0 else GLAM15, g=g)
+    o = O.Oracle(oracle_params(spec, ht=True))
+    rng = np.random.default_rng(seed)
+    Q = o.Q
+    t = O.lattice_tables(model)
+    f = t["omega"] * (0.1 + 0.02 * rng.random((o.N, 1))) * (1 + 0.05 * rng.standard_normal((o.N, Q)))
+    o.set("f", f)
+    o.set("g", t["omega"] * 0.03 * (1 + 0.05 * rng.standard_normal((o.N, Q))))
+    rho = f.sum(1)
+    o.set("rho", rho)
+    o.set("T", np.full(o.N, 0.03))
+    Ft = 1e-4 * rng.standard_normal((o.N, 3))
+    if model == 0:
+        Ft[:, 2] = 0
+    o.set("Ft", Ft)
+    o.set("Fi", Ft * 0.9)
+    o.set("ss", 1e-5 * rng.standard_normal(o.N))
+    return o, t
+
+
+@pytest.mark.parametrize("model,f_model", [(O.D2Q9, O.LIMRT), (O.D3Q15, O.XUMRT)])
+def test_collision_conserves_mass_and_adds_the_force(model, f_model):
+    o, t = _random_case(model, f_model, g=(0.0, 0.0, -1e-4) if model else (0.0, -1e-4, 0.0))
+    o.call("macro_velocity")                    # U = (sum c f + F/2) / rho
+    f0 = o.get("f")
+    rho = o.get("rho")
+    o.call("collision", 0)
+    f1 = o.get("f")
+    c = t["vel"].astype(float)
+    assert np.allclose(f1.sum(1), f0.sum(1), rtol=0, atol=1e-16)
+    if f_model == O.LIMRT:
+        F = o.get("Ft")
+    else:
+        F = o.get("Fi") + (rho - o.rho0)[:, None] * np.array([0.0, 0.0, -1e-4])
+    dj = f1 @ c - f0 @ c
+    assert np.allclose(dj, F, rtol=0, atol=1e-16)
+
+
+@pytest.mark.parametrize("model", [O.D2Q9, O.D3Q15])
+def test_thermal_collision_adds_the_source(model):
+    o, t = _random_case(model, O.LIMRT if model == 0 else O.XUMRT)
+    g0 = o.get("g")
+    o.call("collision", 1)                       # myMRT, Lambda_0 = 0
+    g1 = o.get("g")
+    assert np.allclose(g1.sum(1) - g0.sum(1), o.get("ss"), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("model,f_model", [(O.D2Q9, O.LIMRT), (O.D3Q15, O.XUMRT)])
+def test_rest_equilibrium_is_a_fixed_point(model, f_model):
+    spec = CaseSpec(lbmodel=model, nx=7, ny=7, nz=7 if model else 1, btype=0,
+                    f_model=f_model, f_Lambda=LAM9 if model == 0 else LAM15, g_model=1,
+                    g_Lambda=GLAM9 if model == 0 else GLAM15)
+    from multiphaselb_amd.casefiles import BC
+    for p in spec.patches():
+        spec.f_bc[p] = BC("bounceBack")
+        spec.g_bc[p] = BC("bounceBack")
+    o = O.Oracle(oracle_params(spec, ht=True))
+    o.set("rho", np.full(o.N, 0.12))
+    o.set("T", np.full(o.N, 0.03))
+    o.set("U", np.zeros((o.N, 3)))
+    o.init()
+    F = o.get("Fi")
+    assert np.all(F == 0)                        # uniform density: no interaction force
+    assert np.all(o.get("ss") == 0)              # uniform fields: no heat source
+    f0, g0 = o.get("f"), o.get("g")
+    assert o.run(3) == 3
+    assert np.allclose(o.get("f"), f0, rtol=0, atol=1e-15)
+    assert np.allclose(o.get("g"), g0, rtol=0, atol=1e-15)
+
+
+def test_periodic_stream_is_a_permutation():
+    for model, f_model in ((O.D2Q9, O.LIMRT), (O.D3Q15, O.XUMRT)):
+        o, t = _random_case(model, f_model, nx=7, ny=5, nz=6)
+        nx, ny, nz = 7, 5, (6 if model else 1)
+        f0 = o.get("f").reshape(nz, ny, nx, -1)
+        # the two stored copies of a periodic site must hold the same values
+        f0[:, :, nx - 1] = f0[:, :, 0]
+        f0[:, ny - 1] = f0[:, 0]
+        if model:
+            f0[nz - 1] = f0[0]
+        f0 = f0.reshape(-1, o.Q)
+        o.set("f", f0)
+        o.call("stream", 0)
+        f1 = o.get("f")
+        # each population moved by +c_k on the period-(n-1) lattice of unique sites
+        uniq = lambda a: a.reshape(nz, ny, nx, -1)[: max(nz - 1, 1), : ny - 1, : nx - 1]
+        a0, a1 = uniq(f0), uniq(f1)
+        for k in range(o.Q):
+            cx, cy, cz = t["vel"][k]
+            shifted = np.roll(a0[..., k], shift=(cz, cy, cx) if model else (0, cy, cx), axis=(0, 1, 2))
+            assert np.array_equal(shifted, a1[..., k])
+        assert np.isclose(a1.sum(), a0.sum(), rtol=1e-15)
+
+
+def test_interaction_force_zero_on_uniform_and_mirror_antisymmetric():
+    spec = CaseSpec(lbmodel=0, nx=12, ny=9, btype=3, f_model=0, f_Lambda=LAM9, g_model=None)
+    o = O.Oracle(oracle_params(spec, ht=False))
+    o.set("T", np.full(o.N, 0.03))
+    o.set("rho", np.full(o.N, 0.1))
+    o.call("inter_force")
+    assert np.all(o.get("Fi") == 0)
+    x = np.arange(12) % 11
+    prof = 0.1 + 0.04 * np.sin(2 * np.pi * x / 11.0)
+    rho = np.tile(prof, 9)
+    o.set("rho", rho)
+    o.call("inter_force")
+    F1 = o.get("Fi").reshape(9, 12, 3)
+    mir = np.tile(prof[(11 - x) % 11], 9)
+    o.set("rho", mir)
+    o.call("inter_force")
+    F2 = o.get("Fi").reshape(9, 12, 3)
+    xi = np.arange(11)
+    assert np.allclose(F2[:, (11 - xi) % 11, 0], -F1[:, xi, 0], rtol=1e-12, atol=1e-20)
+    assert np.allclose(F1[..., 1], 0, atol=1e-12 * np.abs(F1).max())
+    # smooth field: F ~ -G psi grad psi (pseudopotential weights sum to the identity)
+    o.set("rho", rho)
+    o.call("inter_force")
+    p = O.make_params(eos=O.VDW, a=0.5, b=4.0)
+    psi = np.sqrt(2 * (np.array([O.lib().orc_p_eos(p, r, 0.03) for r in prof]) - prof / 3) / -1.0)
+    grad = (np.roll(psi, -1) - np.roll(psi, 1)) / 2
+    grad[0] = (psi[1] - psi[10]) / 2
+    grad[10] = (psi[0] - psi[9]) / 2
+    expect = psi[:11] * grad[:11]                 # -G psi grad psi with G = -1
+    got = o.get("Fi").reshape(9, 12, 3)[4, :11, 0]
+    assert np.allclose(got, expect, rtol=0.05, atol=1e-3 * np.abs(expect).max())
+
+
+def test_heat_source_divergence_term_closed_form():
+    """U = (b x, 0, 0), uniform rho and T: S = b T (1 - dp/dT / (rho Cv)) on interior nodes."""
+    spec = CaseSpec(lbmodel=0, nx=9, ny=7, btype=1, f_model=0, f_Lambda=LAM9, g_model=1,
+                    g_Lambda=GLAM9)
+    o = O.Oracle(oracle_params(spec, ht=True))
+    b, rho, T = 1e-4, 0.12, 0.03
+    x = np.tile(np.arange(9.0), 7)
+    U = np.zeros((o.N, 3))
+    U[:, 0] = b * x
+    o.set("U", U)
+    o.set("rho", np.full(o.N, rho))
+    o.set("T", np.full(o.N, T))
+    o.call("heat_source")
+    p = O.make_params(eos=O.VDW, a=0.5, b=4.0)
+    L = O.lib()
+    dpdT = (L.orc_p_eos(p, rho, 1.1 * T) - L.orc_p_eos(p, rho, 0.9 * T)) / (0.2 * T)
+    expect = b * T * (1.0 - dpdT / rho)
+    ss = o.get("ss").reshape(7, 9)
+    assert np.allclose(ss[1:-1, 1:-1], expect, rtol=1e-9)
