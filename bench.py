@@ -111,6 +111,119 @@ def cpu_baseline(args, budget_s=12.0):
                        ("xuMRT+myMRT thermal" if args.thermal else "xuMRT", steps, dt))
 
 
+def laplace(rho, p, L):
+    """Interior/exterior density and pressure (2x2 nodes around the box centre, the box
+    corner), the equivalent radius from the mass on the L-1 periodic sites, and the
+    Laplace product dp * R of one steady case."""
+    r = rho.reshape(L, L)[: L - 1, : L - 1]
+    pp = p.reshape(L, L)
+    c = (L - 1) // 2
+    rin = float(rho.reshape(L, L)[c:c + 2, c:c + 2].mean())
+    pin = float(pp[c:c + 2, c:c + 2].mean())
+    rout, pout = float(r[0, 0]), float(pp[0, 0])
+    area = (float(r.sum()) - rout * (L - 1) ** 2) / (rin - rout)
+    R = float(np.sqrt(max(area, 0.0) / np.pi))
+    return dict(rho_in=rin, rho_out=rout, dp=pin - pout, R=R, gamma=(pin - pout) * R)
+
+
+def run_ensemble(args):
+    """Batched ensemble: n independent D2Q9 liMRT droplet/bubble cases (sizes, radii,
+    temperatures and shapes drawn from a seeded generator) advanced together until each
+    is steady, in FP64 (the reference for the Laplace comparison) and the requested
+    reduced precisions."""
+    import multiphaselb_amd as M
+    from multiphaselb_amd.casefiles import CaseSpec
+    from multiphaselb_amd.initfields import ensemble, ensemble_density
+
+    ens = ensemble(args.cases, seed=7)
+    hosts = {}
+    for c in ens:
+        if c["L"] not in hosts:
+            d = tempfile.mkdtemp(prefix="mplb_ens_")
+            M.write_case(d, CaseSpec(lbmodel=0, nx=c["L"], ny=c["L"], btype=3, f_model=0, a=0.5, b=4.0,
+                                     end=10 ** 8, write_interval=10 ** 8))
+            hosts[c["L"]] = M.Case(d, heat_transfer=False, device=-1)
+    rhos = [ensemble_density(c) for c in ens]
+    nodes = sum(c["L"] ** 2 for c in ens)
+    thr = {"fp64": args.steady_fp64, "fp32": args.steady_fp32, "fp16s": args.steady_fp16s}
+    code = {"fp64": M.FP64, "fp32": M.FP32, "fp16s": M.FP16S}
+    precs = ["fp64"] + [x for x in args.ens_precisions.split(",") if x != "fp64"]
+    res = {}
+    for prec in precs:
+        b = M.Batch([hosts[c["L"]] for c in ens], rhos, [c["T"] for c in ens], precision=code[prec])
+        t0 = time.perf_counter()
+        b.init()
+        block, xcd = b.tune(args.tune_steps)
+        t1 = time.perf_counter()
+        nconv, steps, delta, state = b.run_to_steady(thr[prec], check_every=100, max_steps=args.max_steps)
+        t2 = time.perf_counter()
+        lap = [laplace(b.get(k, "rho"), b.get(k, "p"), c["L"]) if state[k] == 1 else None
+               for k, c in enumerate(ens)]
+        lu = float(sum(c["L"] ** 2 * int(steps[k]) for k, c in enumerate(ens)))
+        res[prec] = dict(steady=int(nconv), diverged=int((state == 2).sum()), threshold=thr[prec],
+                         wall_s=t2 - t0, run_s=t2 - t1, tune_s=t1 - t0, block=block, xcd_order=xcd,
+                         steps_median=float(np.median(steps)), steps_max=int(steps.max()),
+                         mlups=lu / (t2 - t0) / 1e6, lap=lap, state=state)
+        b.close()
+        print("# %s: %d/%d steady, %d diverged, %.1f s (tune %.1f s), block %d xcd %d, steps median %d max %d" % (
+            prec, nconv, len(ens), res[prec]["diverged"], t2 - t0, t1 - t0, block, xcd,
+            res[prec]["steps_median"], res[prec]["steps_max"]), file=sys.stderr, flush=True)
+    ref = res["fp64"]["lap"]
+    tc_bins = np.arange(0.6, 0.951, 0.05)
+    out_prec = {}
+    for prec in precs:
+        r = res[prec]
+        both = [k for k in range(len(ens)) if ref[k] is not None and r["lap"][k] is not None]
+        e = {}
+        for q in ("dp", "gamma", "rho_in", "rho_out", "R"):
+            v = np.array([abs(r["lap"][k][q] - ref[k][q]) / abs(ref[k][q]) for k in both]) if both else np.zeros(1)
+            e[q] = {"median": float(np.median(v)), "max": float(np.max(v))}
+        # Laplace law per T/Tc bin: least-squares dp = s / R + c0
+        fits = []
+        for lo, hi in zip(tc_bins[:-1], tc_bins[1:]):
+            ks = [k for k in both if lo <= ens[k]["tr"] < hi]
+            if len(ks) < 3:
+                continue
+            A = np.array([[1.0 / r["lap"][k]["R"], 1.0] for k in ks])
+            y = np.array([r["lap"][k]["dp"] for k in ks])
+            Ar = np.array([[1.0 / ref[k]["R"], 1.0] for k in ks])
+            yr = np.array([ref[k]["dp"] for k in ks])
+            s = np.linalg.lstsq(A, y, rcond=None)[0][0]
+            sr = np.linalg.lstsq(Ar, yr, rcond=None)[0][0]
+            fits.append({"tr": [round(lo, 2), round(hi, 2)], "cases": len(ks), "slope": float(s),
+                         "rel_err_vs_fp64": float(abs(s - sr) / abs(sr))})
+        out_prec[prec] = {
+            "cases_per_s": round(r["steady"] / r["wall_s"], 3), "steady": r["steady"], "diverged": r["diverged"],
+            "threshold": r["threshold"], "wall_s": round(r["wall_s"], 2), "tune_s": round(r["tune_s"], 2),
+            "mlups": round(r["mlups"], 1), "steps_median": r["steps_median"], "steps_max": r["steps_max"],
+            "launch": {"block": r["block"], "xcd_order": r["xcd_order"]},
+            "laplace_err_vs_fp64": e, "laplace_fit": fits,
+            "laplace_fit_max_rel_err_vs_fp64": max([f["rel_err_vs_fp64"] for f in fits], default=None),
+        }
+    main_prec = precs[-1]
+    out = {
+        "metric": "batched ensemble cases/s to steady state",
+        "value": out_prec[main_prec]["cases_per_s"],
+        "unit": "cases/s",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp16s": "f16-storage/f32"}[main_prec],
+        "data": "synthetic (seeded ensemble, Maxwell densities)",
+        "config": {"workload": "%d D2Q9 liMRT droplet/bubble cases, L in {64,96,128,192,256}, R in [6, L/4], "
+                               "T/Tc in [0.6, 0.95], VdW a=0.5 b=4, seed 7, one batched launch per pass, "
+                               "graph-captured 100-step runs" % len(ens),
+                   "nodes": nodes, "steady": "max|drho|/rho_l over 100 steps < threshold"},
+        "precisions": out_prec,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def copy_probe(mib, reps=5):
     """A plain device-to-device copy of known size; its FETCH_SIZE / WRITE_SIZE counter
     readings give the scale factor applied to the lattice kernels' counters."""
@@ -136,6 +249,14 @@ def main():
     ap.add_argument("--thermal", action="store_true")
     ap.add_argument("--generic", action="store_true", help="per-routine kernels instead of the fused kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="bubble3d", choices=["bubble3d", "ensemble"])
+    ap.add_argument("--cases", type=int, default=1024, help="ensemble: number of cases")
+    ap.add_argument("--ens-precisions", default="fp32,fp16s", help="ensemble: precisions besides fp64")
+    ap.add_argument("--steady-fp64", type=float, default=1e-9)
+    ap.add_argument("--steady-fp32", type=float, default=1e-6)
+    ap.add_argument("--steady-fp16s", type=float, default=1e-4)
+    ap.add_argument("--max-steps", type=int, default=200000)
+    ap.add_argument("--tune-steps", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM traffic counters)")
     args = ap.parse_args()
@@ -145,6 +266,8 @@ def main():
         print("bench.py runs on one GPU: the whole lattice lives on a single MI355X "
               "(multi-GPU sharding is a design note, not built); got --gpus %d" % args.gpus, file=sys.stderr)
         return 2
+    if args.workload == "ensemble":
+        return run_ensemble(args)
 
     r = run_gpu(args)
     N, steps = r["N"], args.steps

@@ -127,6 +127,48 @@ int mplb_lattice_tables(int lbmodel, int* vel, int* rev, double* omega, double* 
  * (fixedRandomT.c:37,81): writes n successive values after srand(seed). */
 int mplb_glibc_rand(unsigned seed, int n, int32_t* out);
 
+/* ---------------------------------------------------------------------------------
+ * Batched ensembles: many independent small cases advanced together on one GPU.
+ * Replaces running multiPhasePP (multiPhasePP.c:157-241) once per case: each case
+ * keeps the reference's time loop with --nht, and the batch advances every case in
+ * the same launches (a work table of (case, node range) per workgroup), captures runs
+ * of steps in a HIP graph and drops each case from the table once it is steady.
+ * Cases: D2Q9 periodicXY boxes, f Collision liMRT without liSurfTen, EOS/rho_0 <= 0,
+ * opened with heat_transfer = 0 and frozen = 0; all cases share the EOS block and the
+ * f model constants (Lambda, sigma); sizes differ freely.  The same case handle may be
+ * passed for several entries (it is read for its mesh and dictionaries only; a
+ * host-only handle, opts.device = -1, suffices).  Temperature is uniform per case.
+ */
+typedef struct mplb_batch mplb_batch;
+int mplb_batch_create(mplb_case* const* cases, int n, int precision, int device, mplb_batch** out);
+int mplb_batch_close(mplb_batch* b);
+const char* mplb_batch_last_error(const mplb_batch* b);
+/* Initial density (node-major, the case's N values) and temperature of case i. */
+int mplb_batch_set_case(mplb_batch* b, int i, const double* rho, size_t n, double T);
+/* equilibrium + first collision of every case (multiPhasePP.c:122-147 and step 1's
+ * collision); afterwards every case is at time level 1. */
+int mplb_batch_init(mplb_batch* b);
+/* n more steps of every case (plain launches). */
+int mplb_batch_step(mplb_batch* b, int n);
+/* Steps the cases still changing in graph-captured runs of check_every (even) steps;
+ * after each run a case whose max_x |rho(x) - rho_prev(x)| / rho_l over the last
+ * check_every steps is below threshold is steady (rho_l = largest initial density of
+ * the case) and leaves the work table, a case whose density turned NaN is diverged.
+ * Stops when no case is left or after max_steps.  Per case (arrays of n, may be NULL):
+ * time level reached, last relative change, state (0 still running, 1 steady,
+ * 2 diverged).  Returns the number of steady cases or a negative MPLB_E* code. */
+int mplb_batch_run_to_steady(mplb_batch* b, double threshold, int check_every, int64_t max_steps,
+                             int64_t* case_steps, double* case_delta, int* case_state);
+/* "rho" or "p" (macroPressure.c:20-65, liMRT form) of case i at its current level. */
+int mplb_batch_get_field(mplb_batch* b, int i, const char* name, double* out, size_t n);
+int64_t mplb_batch_case_steps(const mplb_batch* b, int i);
+/* Autotuner: times nodes per workgroup (64..1024) x XCD-ordered work table (0/1) over
+ * steps_per_trial steps each on the current cases (which advance meanwhile) and keeps
+ * the fastest.  mplb_batch_set_launch sets them by hand. */
+int mplb_batch_tune(mplb_batch* b, int steps_per_trial, int* chosen_block, int* chosen_xcd_order);
+int mplb_batch_set_launch(mplb_batch* b, int block, int xcd_order);
+int mplb_batch_launch_info(const mplb_batch* b, int* block, int* xcd_order, int* workgroups, int64_t* nodes);
+
 #ifdef __cplusplus
 }
 #endif

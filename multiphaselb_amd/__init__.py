@@ -53,6 +53,9 @@ EXPORTS = [
     "mplb_macro_velocity", "mplb_macro_temperature", "mplb_inter_force", "mplb_heat_source",
     "mplb_equilibrium", "mplb_macro_pressure", "mplb_average_density", "mplb_time_steps",
     "mplb_timer_name", "mplb_lattice_tables", "mplb_glibc_rand",
+    "mplb_batch_create", "mplb_batch_close", "mplb_batch_last_error", "mplb_batch_set_case",
+    "mplb_batch_init", "mplb_batch_step", "mplb_batch_run_to_steady", "mplb_batch_get_field",
+    "mplb_batch_case_steps", "mplb_batch_tune", "mplb_batch_set_launch", "mplb_batch_launch_info",
 ]
 
 
@@ -92,6 +95,18 @@ def library():
         "mplb_timer_name": ([i], cp),
         "mplb_lattice_tables": ([i, vp, vp, vp, vp, vp, vp], i),
         "mplb_glibc_rand": ([C.c_uint, i, vp], i),
+        "mplb_batch_create": ([vp, i, i, i, C.POINTER(vp)], i),
+        "mplb_batch_close": ([vp], i),
+        "mplb_batch_last_error": ([vp], cp),
+        "mplb_batch_set_case": ([vp, i, vp, C.c_size_t, d], i),
+        "mplb_batch_init": ([vp], i),
+        "mplb_batch_step": ([vp, i], i),
+        "mplb_batch_run_to_steady": ([vp, d, i, C.c_int64, vp, vp, vp], i),
+        "mplb_batch_get_field": ([vp, i, cp, vp, C.c_size_t], i),
+        "mplb_batch_case_steps": ([vp, i], C.c_int64),
+        "mplb_batch_tune": ([vp, i, C.POINTER(i), C.POINTER(i)], i),
+        "mplb_batch_set_launch": ([vp, i, i], i),
+        "mplb_batch_launch_info": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(C.c_int64)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -249,3 +264,83 @@ class Case:
 
     def macro_pressure(self):
         self._check(self.L.mplb_macro_pressure(self.h))
+
+
+class Batch:
+    """Ensemble of D2Q9 periodic liMRT cases advanced together (mplb_batch_*).
+
+    `cases` are Case handles (host-only ones, device=-1, are enough; one handle may be
+    listed for several entries); `rho` and `T` give each entry's initial density (N
+    values) and uniform temperature."""
+
+    def __init__(self, cases, rho, T, precision=FP32, device=0):
+        self.L = library()
+        self.cases = list(cases)
+        self.n = len(self.cases)
+        arr = (C.c_void_p * self.n)(*[c.h.value for c in self.cases])
+        h = C.c_void_p()
+        rc = self.L.mplb_batch_create(arr, self.n, precision, device, C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.L.mplb_batch_last_error(h).decode() if h.value else "batch create failed"
+            self.close()
+            raise MplbError(rc, msg)
+        self.sizes = [c.N for c in self.cases]
+        for k in range(self.n):
+            self.set_case(k, rho[k], T[k])
+
+    def set_case(self, k, rho, T):
+        r = np.ascontiguousarray(rho, dtype=np.float64).ravel()
+        self._check(self.L.mplb_batch_set_case(self.h, k, r.ctypes.data, r.size, float(T)))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.mplb_batch_close(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise MplbError(rc, self.L.mplb_batch_last_error(self.h).decode())
+        return rc
+
+    def init(self):
+        self._check(self.L.mplb_batch_init(self.h))
+
+    def step(self, n):
+        self._check(self.L.mplb_batch_step(self.h, n))
+
+    def run_to_steady(self, threshold, check_every=100, max_steps=10 ** 6):
+        steps = np.zeros(self.n, np.int64)
+        delta = np.zeros(self.n)
+        state = np.zeros(self.n, np.int32)
+        nconv = self._check(self.L.mplb_batch_run_to_steady(self.h, threshold, check_every, max_steps,
+                                                             steps.ctypes.data, delta.ctypes.data,
+                                                             state.ctypes.data))
+        return nconv, steps, delta, state
+
+    def get(self, k, name="rho"):
+        a = np.zeros(self.sizes[k])
+        self._check(self.L.mplb_batch_get_field(self.h, k, name.encode(), a.ctypes.data, a.size))
+        return a
+
+    def case_steps(self, k):
+        return int(self.L.mplb_batch_case_steps(self.h, k))
+
+    def tune(self, steps_per_trial=20):
+        b, x = C.c_int(), C.c_int()
+        self._check(self.L.mplb_batch_tune(self.h, steps_per_trial, C.byref(b), C.byref(x)))
+        return b.value, x.value
+
+    def set_launch(self, block, xcd_order):
+        self._check(self.L.mplb_batch_set_launch(self.h, block, 1 if xcd_order else 0))
+
+    def launch_info(self):
+        b, x, w, n = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
+        self._check(self.L.mplb_batch_launch_info(self.h, C.byref(b), C.byref(x), C.byref(w), C.byref(n)))
+        return dict(block=b.value, xcd_order=x.value, workgroups=w.value, nodes=n.value)

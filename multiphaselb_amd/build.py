@@ -28,11 +28,13 @@ COMMON = ["-std=c++17", "-fPIC", "-I" + CSRC, "-I" + os.path.join(ROOT, "include
 HIP_DEV = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-Wno-unused-command-line-argument"]
 
-# (source, flags): the parity kernels and all host code keep plain IEEE arithmetic
-# (no fused multiply-add contraction); the fused FP32/FP16 kernels may contract.
+# (source, flags): the parity kernels, the batched-ensemble kernels (whose FP64 mode is a
+# parity mode) and all host code keep plain IEEE arithmetic (no fused multiply-add
+# contraction); the fused FP32/FP16 kernels may contract.
 UNITS = [
     ("kernels_generic.hip", ["-O3", "-ffp-contract=off"] + HIP_DEV),
     ("kernels_fused.hip", ["-O3", "-ffp-contract=fast"] + HIP_DEV),
+    ("kernels_batch.hip", ["-O3", "-ffp-contract=off"] + HIP_DEV),
     ("capi.hip", ["-O2", "-ffp-contract=off"] + HIP_DEV),
     ("dict.cpp", ["-O2", "-ffp-contract=off"]),
     ("mesh.cpp", ["-O2", "-ffp-contract=off"]),

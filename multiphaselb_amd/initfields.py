@@ -121,3 +121,33 @@ def pool(shape, height, below, above, width, axis=0):
     s = [1, 1, 1]
     s[axis] = shape[axis]
     return np.broadcast_to(prof.reshape(s), shape).copy()
+
+
+def ensemble(n, seed=7, sizes=(64, 96, 128, 192, 256), tr_range=(0.6, 0.95), a=0.5, b=4.0,
+             width=5.0, amp=1e-4, noise_seed=20240601):
+    """Parameters of n independent D2Q9 droplet/bubble cases on L x L periodic boxes:
+    L uniform over `sizes`, radius uniform in [6, L/4], T/Tc uniform in tr_range, droplet
+    or bubble with probability 1/2 each, all from one splitmix64 stream (seed).  Each
+    case's noise uses seed noise_seed + case index."""
+    u = splitmix64(seed, 4 * n).reshape(n, 4)
+    tc = vdw_critical(a, b)[1]
+    out = []
+    for k in range(n):
+        L = int(sizes[min(int(u[k, 0] * len(sizes)), len(sizes) - 1)])
+        R = 6.0 + u[k, 1] * (L / 4.0 - 6.0)
+        tr = tr_range[0] + u[k, 2] * (tr_range[1] - tr_range[0])
+        rv, rl = maxwell(tr * tc, a=a, b=b)
+        out.append(dict(index=k, L=L, radius=R, tr=tr, T=tr * tc, droplet=bool(u[k, 3] < 0.5),
+                        rho_v=rv, rho_l=rl, width=width, amp=amp, noise_seed=noise_seed + k))
+    return out
+
+
+def ensemble_density(case):
+    """Initial density (node-major, i fastest) of one ensemble case: tanh profile
+    between the coexistence densities around the box centre, plus noise."""
+    L = case["L"]
+    inside, outside = (case["rho_l"], case["rho_v"]) if case["droplet"] else (case["rho_v"], case["rho_l"])
+    c = (L - 1) / 2.0
+    rho = sphere((1, L, L), (c, c, 0.0), case["radius"], inside, outside, case["width"])
+    rho = rho + noise((1, L, L), case["amp"], case["noise_seed"], (True, True, False))
+    return rho.ravel()
