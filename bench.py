@@ -224,6 +224,173 @@ def run_ensemble(args):
     return 0
 
 
+def run_to_steady_case(case, rho_l, thr, check_every, max_steps):
+    """Steps one case until max|drho|/rho_l over check_every steps < thr; returns
+    (steps, wall seconds, steady flag)."""
+    t0 = time.perf_counter()
+    case.density_change()
+    steps = 0
+    while steps < max_steps:
+        case.step(check_every)
+        steps += check_every
+        d = case.density_change()
+        if not np.isfinite(d):
+            return steps, time.perf_counter() - t0, False
+        if d / rho_l < thr:
+            return steps, time.perf_counter() - t0, True
+    return steps, time.perf_counter() - t0, False
+
+
+def bubble3d_cases(n, seed, L, droplet_share):
+    from multiphaselb_amd.initfields import splitmix64, vdw_critical, maxwell
+
+    u = splitmix64(seed, 3 * n).reshape(n, 3)
+    tc = vdw_critical(0.5, 4.0)[1]
+    out = []
+    for k in range(n):
+        R = 6.0 + u[k, 0] * (L / 4.0 - 6.0)
+        tr = 0.6 + 0.35 * u[k, 1]
+        rv, rl = maxwell(tr * tc, a=0.5, b=4.0)
+        out.append(dict(L=L, radius=R, tr=tr, T=tr * tc, droplet=bool(u[k, 2] < droplet_share), rho_v=rv, rho_l=rl,
+                        noise_seed=20240601 + seed * 100000 + k))
+    return out
+
+
+def run_warmstart(args):
+    """Learned warm start: solver-generated training profiles (2D ensemble cases and 3D
+    cases), GPU training of the network, then held-out 3D bubbles run cold and warm to
+    the same steady criterion."""
+    import multiphaselb_amd as M
+    from multiphaselb_amd import warmstart as W
+    from multiphaselb_amd.casefiles import CaseSpec
+    from multiphaselb_amd.initfields import ensemble, ensemble_density, noise, sphere
+
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+    L3 = args.ws_size
+    thr = args.steady_fp32
+    samples_X, samples_Y = [], []
+    t_data = time.perf_counter()
+    # 2D cases from the ensemble generator, advanced together in one batch
+    if args.ws_2d:
+        ens = ensemble(args.ws_2d, seed=11)
+        hosts = {}
+        for c in ens:
+            if c["L"] not in hosts:
+                d = tempfile.mkdtemp(prefix="mplb_ws2_")
+                M.write_case(d, CaseSpec(lbmodel=0, nx=c["L"], ny=c["L"], btype=3, f_model=0, a=0.5, b=4.0,
+                                         end=10 ** 8, write_interval=10 ** 8))
+                hosts[c["L"]] = M.Case(d, heat_transfer=False, device=-1)
+        b = M.Batch([hosts[c["L"]] for c in ens], [ensemble_density(c) for c in ens], [c["T"] for c in ens],
+                    precision=M.FP32)
+        b.init()
+        b.tune(10)
+        nconv, steps, delta, state = b.run_to_steady(thr, check_every=100, max_steps=args.max_steps)
+        for k, c in enumerate(ens):
+            if state[k] != 1:
+                continue
+            L = c["L"]
+            r = W.node_radius((L, L), ((L - 1) / 2.0, (L - 1) / 2.0))
+            y, _ = W.extract_profile(b.get(k, "rho"), r, c["rho_v"], c["rho_l"], c["droplet"])
+            samples_X.append(W.features(c["tr"], c["radius"], L, c["droplet"], 2))
+            samples_Y.append(np.concatenate([y, np.zeros(W.NPROF)]))
+        b.close()
+        log("2D: %d/%d steady in %.1f s, steps median %d" % (nconv, len(ens), time.perf_counter() - t_data,
+                                                            int(np.median(steps))))
+    d3 = tempfile.mkdtemp(prefix="mplb_ws3_")
+    M.write_case(d3, CaseSpec(lbmodel=1, nx=L3, ny=L3, nz=L3, btype=5, f_model=0, f_Lambda=LAM15, f_sigma=0.125,
+                              g_model=1, g_Lambda=GLAM15, end=10 ** 8, write_interval=10 ** 8))
+    case = M.Case(d3, precision=M.FP32, heat_transfer=False, fused=True, extensions=M.EXT_LIMRT_D3Q15)
+    N = case.N
+    cen = ((L3 - 1) / 2.0,) * 3
+    r3 = W.node_radius((L3, L3, L3), cen)
+
+    def cold_field(c):
+        inside, outside = (c["rho_l"], c["rho_v"]) if c["droplet"] else (c["rho_v"], c["rho_l"])
+        rho = sphere((L3, L3, L3), cen, c["radius"], inside, outside, 5.0) + noise((L3, L3, L3), 1e-4, c["noise_seed"])
+        return rho.ravel()
+
+    def start(rho=None):
+        if rho is not None:
+            case.set("rho", rho)
+        case.set("U", np.zeros((N, 3)))
+        case.set("T", np.full(N, c["T"]))
+        case.init()
+
+    t3 = time.perf_counter()
+    train3 = bubble3d_cases(args.ws_3d, 13, L3, 0.5)
+    for i, c in enumerate(train3):
+        start(cold_field(c))
+        st, wall, ok = run_to_steady_case(case, c["rho_l"], thr, 100, args.max_steps)
+        if ok:
+            y, _ = W.extract_profile(case.get("rho"), r3, c["rho_v"], c["rho_l"], c["droplet"])
+            samples_X.append(W.features(c["tr"], c["radius"], L3, c["droplet"], 3))
+            samples_Y.append(np.concatenate([y, np.zeros(W.NPROF)]))
+        if i % 16 == 0:
+            log("3D train case %d: %d steps %.2f s steady %s" % (i, st, wall, ok))
+    log("3D training runs: %.1f s" % (time.perf_counter() - t3))
+    X = np.array(samples_X, np.float32)
+    Y = np.array(samples_Y, np.float32)
+    t_train = time.perf_counter()
+    net = M.Mlp(W.WIDTHS, args.ws_batch)
+    net.set_params(W.init_params(W.WIDTHS, 0))
+    hist = W.train(net, X, Y, iters=args.ws_iters, seed=1)
+    t_train = time.perf_counter() - t_train
+    log("trained %d samples, %d params, %d iters in %.1f s, loss %.3e -> %.3e" % (
+        X.shape[0], net.P, args.ws_iters, t_train, hist[:20].mean(), hist[-50:].mean()))
+    evals = bubble3d_cases(args.ws_eval, 17, L3, 0.0)
+    rows = []
+    for i, c in enumerate(evals):
+        rho0 = cold_field(c)
+        mass = float(rho0.sum())
+        start(rho0)
+        st_c, wall_c, ok_c = run_to_steady_case(case, c["rho_l"], thr, 100, args.max_steps)
+        rc = case.get("rho")
+        tw = time.perf_counter()
+        y = net.forward(W.features(c["tr"], c["radius"], L3, c["droplet"], 3)[None, :])[0, :W.NPROF]
+        prof = W.profile_density(y, c["rho_v"], c["rho_l"])
+        W.raster_case(case, prof, cen, mass, c["radius"])
+        start(None)
+        t_prop = time.perf_counter() - tw
+        st_w, wall_w, ok_w = run_to_steady_case(case, c["rho_l"], thr, 100, args.max_steps)
+        wall_w += t_prop
+        rw = case.get("rho")
+        cin, cout = rc[r3 < 2].mean(), rc[r3 > r3.max() - 2].mean()
+        win, wout = rw[r3 < 2].mean(), rw[r3 > r3.max() - 2].mean()
+        dev = max(abs(win - cin) / cin, abs(wout - cout) / cout)
+        rows.append(dict(steps_cold=st_c, steps_warm=st_w, wall_cold=wall_c, wall_warm=wall_w, ok=ok_c and ok_w,
+                         dev=dev, prop_s=t_prop))
+        if i % 16 == 0:
+            log("eval %d: cold %d steps %.2f s, warm %d steps %.2f s (proposal %.3f s), coexistence dev %.1e" % (
+                i, st_c, wall_c, st_w, wall_w, t_prop, dev))
+    ok = [r for r in rows if r["ok"]]
+    sp_steps = np.array([r["steps_cold"] / r["steps_warm"] for r in ok])
+    sp_wall = np.array([r["wall_cold"] / r["wall_warm"] for r in ok])
+    devs = np.array([r["dev"] for r in ok])
+    out = {
+        "metric": "time-to-steady-state NN-warm vs plain",
+        "value": round(float(np.sum([r["wall_cold"] for r in ok]) / np.sum([r["wall_warm"] for r in ok])), 3)
+        if ok else None,
+        "unit": "x (total cold wall time / total warm wall time)",
+        "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (solver), bf16 MFMA / f32 accumulate (network)",
+        "data": "synthetic: training profiles from solver runs (2D ensemble + 3D cases), held-out 3D bubbles",
+        "config": {"workload": "D3Q15 liMRT static bubbles %d^3 FP32, R in [6, %d], T/Tc in [0.6, 0.95]" % (L3, L3 // 4),
+                   "steady": "max|drho|/rho_l over 100 steps < %g" % thr, "train_2d": args.ws_2d,
+                   "train_3d": args.ws_3d, "eval": len(evals), "network": {"widths": W.WIDTHS, "params": net.P,
+                                                                             "iters": args.ws_iters, "batch": args.ws_batch}},
+        "speedup_steps": {"median": float(np.median(sp_steps)), "min": float(sp_steps.min()),
+                          "max": float(sp_steps.max())} if ok else None,
+        "speedup_wall": {"median": float(np.median(sp_wall)), "min": float(sp_wall.min()),
+                         "max": float(sp_wall.max())} if ok else None,
+        "coexistence_rel_dev": {"median": float(np.median(devs)), "max": float(devs.max()),
+                                "failures_over_1e-4": int((devs > 1e-4).sum())} if ok else None,
+        "steady_both": len(ok), "train_samples": int(X.shape[0]), "train_s": round(t_train, 2),
+        "data_s": round(time.perf_counter() - t_data, 1),
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def copy_probe(mib, reps=5):
     """A plain device-to-device copy of known size; its FETCH_SIZE / WRITE_SIZE counter
     readings give the scale factor applied to the lattice kernels' counters."""
@@ -249,7 +416,13 @@ def main():
     ap.add_argument("--thermal", action="store_true")
     ap.add_argument("--generic", action="store_true", help="per-routine kernels instead of the fused kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="bubble3d", choices=["bubble3d", "ensemble"])
+    ap.add_argument("--workload", default="bubble3d", choices=["bubble3d", "ensemble", "warmstart"])
+    ap.add_argument("--ws-2d", type=int, default=4096, help="warmstart: 2D training cases")
+    ap.add_argument("--ws-3d", type=int, default=256, help="warmstart: 3D training cases")
+    ap.add_argument("--ws-eval", type=int, default=256, help="warmstart: held-out 3D bubbles")
+    ap.add_argument("--ws-size", type=int, default=128)
+    ap.add_argument("--ws-iters", type=int, default=4000)
+    ap.add_argument("--ws-batch", type=int, default=256)
     ap.add_argument("--cases", type=int, default=1024, help="ensemble: number of cases")
     ap.add_argument("--ens-precisions", default="fp32,fp16s", help="ensemble: precisions besides fp64")
     ap.add_argument("--steady-fp64", type=float, default=1e-9)
@@ -268,6 +441,8 @@ def main():
         return 2
     if args.workload == "ensemble":
         return run_ensemble(args)
+    if args.workload == "warmstart":
+        return run_warmstart(args)
 
     r = run_gpu(args)
     N, steps = r["N"], args.steps

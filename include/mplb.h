@@ -169,6 +169,48 @@ int mplb_batch_tune(mplb_batch* b, int steps_per_trial, int* chosen_block, int* 
 int mplb_batch_set_launch(mplb_batch* b, int block, int xcd_order);
 int mplb_batch_launch_info(const mplb_batch* b, int* block, int* xcd_order, int* workgroups, int64_t* nodes);
 
+/* ---------------------------------------------------------------------------------
+ * Runs to steady state and the learned warm start.  No counterpart in the reference
+ * (its cases start from setInitialFields generators, setInitialFields.c:131-200, and
+ * run a fixed number of steps); these calls let a driver stop at a steady criterion
+ * and start a case from a proposed near-steady density.
+ */
+/* max over nodes of |rho - rho at the previous call| for the current time level (device
+ * reduction; NaN if a density is NaN).  The first call after mplb_init only takes the
+ * snapshot and returns NaN. */
+int mplb_density_change(mplb_case* c, double* max_abs);
+/* rho(x) = prof(|x - centre| - R) on the stored lattice (x = node coordinates), prof
+ * sampled at d0 + k dd, k = 0..nprof-1, linear interpolation, clamped at both ends;
+ * writes the case's density (call mplb_init afterwards) and returns the total mass of
+ * the stored field in *mass. */
+int mplb_set_radial_density(mplb_case* c, const double* prof, int nprof, double d0, double dd, const double* centre,
+                            double R, double* mass);
+
+/* Multilayer perceptron (tanh hidden layers, linear output, mean squared error, Adam)
+ * whose products run on the matrix cores with BF16 operands and FP32 accumulation;
+ * FP32 master parameters.  widths[0..nlayers]: input width (<= 32), then layer widths
+ * (multiples of 32).  Parameters are laid out per layer as W [in][out] then b [out].
+ * Every product processes `batch` rows (a multiple of 32). */
+typedef struct mplb_mlp mplb_mlp;
+int mplb_mlp_create(int nlayers, const int* widths, int batch, int device, mplb_mlp** out);
+int mplb_mlp_close(mplb_mlp* n);
+const char* mplb_mlp_last_error(const mplb_mlp* n);
+int64_t mplb_mlp_num_params(const mplb_mlp* n);
+/* also resets the Adam moments */
+int mplb_mlp_set_params(mplb_mlp* n, const float* params, size_t count);
+int mplb_mlp_get_params(mplb_mlp* n, float* params, size_t count);
+/* y [rows][out] = net(x [rows][in]) for any number of rows */
+int mplb_mlp_forward(mplb_mlp* n, const float* x, int rows, float* y);
+/* loss and gradient of one batch (x [batch][in], t [batch][out]); grad may be NULL */
+int mplb_mlp_gradient(mplb_mlp* n, const float* x, const float* t, float* loss, float* grad, size_t ngrad);
+/* iters Adam steps on minibatches idx[it * batch .. +batch) of the samples X/T, replayed
+ * from one captured graph; loss_hist[it] (may be NULL) is each minibatch's loss. */
+int mplb_mlp_train(mplb_mlp* n, const float* X, const float* T, int nsamples, const int32_t* idx, int iters,
+                   float lr, float beta1, float beta2, float eps, float* loss_hist);
+/* intermediate tensors of the last gradient call as FP32: "a" (level j BF16 input/
+ * activation), "a32", "y", "dz", "dzb", "wb" (layer j BF16 weights [in_padded][out]) */
+int mplb_mlp_tensor(mplb_mlp* n, const char* name, int j, float* out, size_t count);
+
 #ifdef __cplusplus
 }
 #endif

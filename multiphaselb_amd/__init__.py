@@ -56,6 +56,9 @@ EXPORTS = [
     "mplb_batch_create", "mplb_batch_close", "mplb_batch_last_error", "mplb_batch_set_case",
     "mplb_batch_init", "mplb_batch_step", "mplb_batch_run_to_steady", "mplb_batch_get_field",
     "mplb_batch_case_steps", "mplb_batch_tune", "mplb_batch_set_launch", "mplb_batch_launch_info",
+    "mplb_density_change", "mplb_set_radial_density",
+    "mplb_mlp_create", "mplb_mlp_close", "mplb_mlp_last_error", "mplb_mlp_num_params", "mplb_mlp_set_params",
+    "mplb_mlp_get_params", "mplb_mlp_forward", "mplb_mlp_gradient", "mplb_mlp_train", "mplb_mlp_tensor",
 ]
 
 
@@ -107,6 +110,18 @@ def library():
         "mplb_batch_tune": ([vp, i, C.POINTER(i), C.POINTER(i)], i),
         "mplb_batch_set_launch": ([vp, i, i], i),
         "mplb_batch_launch_info": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(C.c_int64)], i),
+        "mplb_density_change": ([vp, C.POINTER(d)], i),
+        "mplb_set_radial_density": ([vp, vp, i, d, d, vp, d, C.POINTER(d)], i),
+        "mplb_mlp_create": ([i, vp, i, i, C.POINTER(vp)], i),
+        "mplb_mlp_close": ([vp], i),
+        "mplb_mlp_last_error": ([vp], cp),
+        "mplb_mlp_num_params": ([vp], C.c_int64),
+        "mplb_mlp_set_params": ([vp, vp, C.c_size_t], i),
+        "mplb_mlp_get_params": ([vp, vp, C.c_size_t], i),
+        "mplb_mlp_forward": ([vp, vp, i, vp], i),
+        "mplb_mlp_gradient": ([vp, vp, vp, C.POINTER(C.c_float), vp, C.c_size_t], i),
+        "mplb_mlp_train": ([vp, vp, vp, i, vp, i, C.c_float, C.c_float, C.c_float, C.c_float, vp], i),
+        "mplb_mlp_tensor": ([vp, cp, i, vp, C.c_size_t], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -217,6 +232,21 @@ class Case:
         done = C.c_int(0)
         self._check(self.L.mplb_step(self.h, n, C.byref(done)))
         return done.value
+
+    def density_change(self):
+        """max |rho - rho at the previous call| (NaN on the first call after init)."""
+        v = C.c_double()
+        self._check(self.L.mplb_density_change(self.h, C.byref(v)))
+        return v.value
+
+    def set_radial_density(self, prof, d0, dd, centre, R):
+        """Writes rho = prof(|x - centre| - R) into the case; returns the total mass."""
+        pr = np.ascontiguousarray(prof, dtype=np.float64)
+        cen = np.ascontiguousarray(list(centre) + [0.0] * (3 - len(centre)), dtype=np.float64)
+        m = C.c_double()
+        self._check(self.L.mplb_set_radial_density(self.h, pr.ctypes.data, pr.size, d0, dd, cen.ctypes.data, R,
+                                                    C.byref(m)))
+        return m.value
 
     def average_density(self):
         v = C.c_double()
@@ -344,3 +374,75 @@ class Batch:
         b, x, w, n = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
         self._check(self.L.mplb_batch_launch_info(self.h, C.byref(b), C.byref(x), C.byref(w), C.byref(n)))
         return dict(block=b.value, xcd_order=x.value, workgroups=w.value, nodes=n.value)
+
+
+class Mlp:
+    """BF16-MFMA multilayer perceptron on the GPU (mplb_mlp_*)."""
+
+    def __init__(self, widths, batch, device=0):
+        self.L = library()
+        self.widths = list(widths)
+        self.batch = batch
+        w = np.ascontiguousarray(self.widths, dtype=np.int32)
+        h = C.c_void_p()
+        rc = self.L.mplb_mlp_create(len(widths) - 1, w.ctypes.data, batch, device, C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.L.mplb_mlp_last_error(h).decode() if h.value else "network create failed"
+            self.close()
+            raise MplbError(rc, msg)
+        self.P = int(self.L.mplb_mlp_num_params(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.mplb_mlp_close(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise MplbError(rc, self.L.mplb_mlp_last_error(self.h).decode())
+        return rc
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self.L.mplb_mlp_set_params(self.h, a.ctypes.data, a.size))
+
+    def get_params(self):
+        a = np.zeros(self.P, np.float32)
+        self._check(self.L.mplb_mlp_get_params(self.h, a.ctypes.data, a.size))
+        return a
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.widths[-1]), np.float32)
+        self._check(self.L.mplb_mlp_forward(self.h, x.ctypes.data, x.shape[0], y.ctypes.data))
+        return y
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_float()
+        g = np.zeros(self.P, np.float32)
+        self._check(self.L.mplb_mlp_gradient(self.h, x.ctypes.data, t.ctypes.data, C.byref(loss), g.ctypes.data, g.size))
+        return float(loss.value), g
+
+    def train(self, X, T, idx, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        iters = idx.size // self.batch
+        hist = np.zeros(iters, np.float32)
+        self._check(self.L.mplb_mlp_train(self.h, X.ctypes.data, T.ctypes.data, X.shape[0], idx.ctypes.data, iters,
+                                          lr, beta1, beta2, eps, hist.ctypes.data))
+        return hist
+
+    def tensor(self, name, j, shape):
+        a = np.zeros(shape, np.float32)
+        self._check(self.L.mplb_mlp_tensor(self.h, name.encode(), j, a.ctypes.data, a.size))
+        return a

@@ -35,6 +35,8 @@ UNITS = [
     ("kernels_generic.hip", ["-O3", "-ffp-contract=off"] + HIP_DEV),
     ("kernels_fused.hip", ["-O3", "-ffp-contract=fast"] + HIP_DEV),
     ("kernels_batch.hip", ["-O3", "-ffp-contract=off"] + HIP_DEV),
+    ("mlp.hip", ["-O3", "-ffp-contract=off"] + HIP_DEV),
+    ("warm.hip", ["-O3", "-ffp-contract=off"] + HIP_DEV),
     ("capi.hip", ["-O2", "-ffp-contract=off"] + HIP_DEV),
     ("dict.cpp", ["-O2", "-ffp-contract=off"]),
     ("mesh.cpp", ["-O2", "-ffp-contract=off"]),

@@ -678,6 +678,7 @@ int init_t(mplb_case* c) {
     heat_source_t<T>(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->initialized = true;
+    c->snap_valid = false;
     if (c->fused) return fused_after_init(c);
     return 0;
 }
@@ -787,7 +788,7 @@ int mplb_case_close(mplb_case* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->fused) fused_destroy(c);
     void* ptrs[] = {c->f[0], c->f[1], c->f_alt[0], c->f_alt[1], c->rho, c->temp, c->p, c->U, c->Fi, c->Ft, c->ss, c->psi,
-                    c->nb, c->onbnd, c->rnd, c->red, c->nanc};
+                    c->nb, c->onbnd, c->rnd, c->red, c->nanc, c->rho_snap};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (auto& b : c->bd) {

@@ -75,6 +75,8 @@ struct mplb_case {
     double* rnd = nullptr;                      // fixedRandomT draws r = rand()/RAND_MAX
     double* red = nullptr;                      // reduction result + partials
     unsigned int* nanc = nullptr;
+    void* rho_snap = nullptr;                   // density snapshot of mplb_density_change
+    bool snap_valid = false;
 
     // fused structured-box path
     bool fused = false;

@@ -1,0 +1,171 @@
+// Case-level helpers of the learned warm start and of runs to steady state, on the
+// case's own device fields:
+//   mplb_set_radial_density  rho(x) = profile(|x - centre| - R) (linear interpolation,
+//                            clamped ends) written straight into the case's density,
+//                            returning the total mass, so the host can pick R so the
+//                            proposal carries the mass of the cold initial field;
+//   mplb_density_change      max_x |rho(x) - rho_prev(x)| of the current time level
+//                            against the previous call's snapshot (device reduction).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/mplb.h"
+#include "case.h"
+#include "fused.h"
+
+using namespace mplb;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+int wfail(mplb_case* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    return code;
+}
+
+#define WCHK(c, call)                                                                                           \
+    do {                                                                                                        \
+        hipError_t _e = (call);                                                                                 \
+        if (_e != hipSuccess) return wfail((c), MPLB_E_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+struct Radial {
+    int64_t N;
+    int nx, ny;
+    double cx, cy, cz, R, d0, dd;
+    const double* prof;
+    int nprof;
+};
+
+// one partial mass per block (FP64, fixed order); the host adds the partials in order
+template <typename T>
+__global__ void k_radial(Radial a, T* __restrict__ rho, double* __restrict__ part) {
+    __shared__ double s[kBlock];
+    const int64_t id = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    double v = 0.0;
+    if (id < a.N) {
+        const int64_t nxy = (int64_t)a.nx * a.ny;
+        const int64_t k = id / nxy, r = id - k * nxy;
+        const int64_t j = r / a.nx, i = r - j * a.nx;
+        const double dx = (double)i - a.cx, dy = (double)j - a.cy, dz = (double)k - a.cz;
+        const double t = (sqrt(dx * dx + dy * dy + dz * dz) - a.R - a.d0) / a.dd;
+        double x;
+        if (t <= 0.0) {
+            x = a.prof[0];
+        } else if (t >= (double)(a.nprof - 1)) {
+            x = a.prof[a.nprof - 1];
+        } else {
+            const int q = (int)t;
+            const double w = t - (double)q;
+            x = a.prof[q] + w * (a.prof[q + 1] - a.prof[q]);
+        }
+        const T xv = (T)x;
+        rho[id] = xv;
+        v = (double)xv;
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// per-block max of |x - snap| as the bit pattern of a non-negative double (a NaN
+// difference orders above every number), then snap := x
+template <typename T>
+__global__ void k_change(const T* __restrict__ x, T* __restrict__ snap, int64_t n, unsigned long long* part) {
+    __shared__ unsigned long long s[kBlock];
+    const int64_t id = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    unsigned long long u = 0;
+    if (id < n) {
+        const double d = fabs((double)x[id] - (double)snap[id]);
+        u = (unsigned long long)__double_as_longlong(d);
+        snap[id] = x[id];
+    }
+    s[threadIdx.x] = u;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] > s[threadIdx.x + h] ? s[threadIdx.x] : s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+}  // namespace
+
+extern "C" {
+
+int mplb_set_radial_density(mplb_case* c, const double* prof, int nprof, double d0, double dd, const double* centre,
+                            double R, double* mass) {
+    if (!c || !prof || nprof < 2 || !(dd > 0) || !centre) return MPLB_E_ARG;
+    if (c->opts.device < 0 || !c->rho) return wfail(c, MPLB_E_STATE, "the case has no device fields (device < 0)");
+    WCHK(c, hipSetDevice(c->opts.device));
+    const int64_t N = c->mesh.N;
+    const unsigned nb = (unsigned)((N + kBlock - 1) / kBlock);
+    double* dprof = nullptr;
+    double* part = nullptr;
+    WCHK(c, hipMalloc(&dprof, sizeof(double) * nprof));
+    WCHK(c, hipMalloc(&part, sizeof(double) * nb));
+    WCHK(c, hipMemcpyAsync(dprof, prof, sizeof(double) * nprof, hipMemcpyHostToDevice, c->stream));
+    Radial a{N, c->mesh.nx, c->mesh.ny, centre[0], centre[1], c->mesh.d == 3 ? centre[2] : 0.0, R, d0, dd, dprof, nprof};
+    if (c->scalar_bytes == 8) k_radial<double><<<nb, kBlock, 0, c->stream>>>(a, (double*)c->rho, part);
+    else k_radial<float><<<nb, kBlock, 0, c->stream>>>(a, (float*)c->rho, part);
+    WCHK(c, hipGetLastError());
+    std::string hostbuf(sizeof(double) * nb, '\0');
+    double* hp = (double*)&hostbuf[0];
+    WCHK(c, hipMemcpyAsync(hp, part, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+    WCHK(c, hipStreamSynchronize(c->stream));
+    WCHK(c, hipFree(dprof));
+    WCHK(c, hipFree(part));
+    double s = 0;
+    for (unsigned b = 0; b < nb; b++) s += hp[b];
+    if (mass) *mass = s;
+    c->initialized = false;       // the new density takes effect at mplb_init
+    return 0;
+}
+
+int mplb_density_change(mplb_case* c, double* max_abs) {
+    if (!c || !max_abs) return MPLB_E_ARG;
+    if (c->opts.device < 0 || !c->rho) return wfail(c, MPLB_E_STATE, "the case has no device fields (device < 0)");
+    if (!c->initialized) return wfail(c, MPLB_E_STATE, "mplb_init first");
+    WCHK(c, hipSetDevice(c->opts.device));
+    if (c->fused) {
+        const int rc = fused_sync_to_generic(c);
+        if (rc) return rc;
+    }
+    const int64_t N = c->mesh.N;
+    const unsigned nb = (unsigned)((N + kBlock - 1) / kBlock);
+    if (!c->rho_snap) WCHK(c, hipMalloc(&c->rho_snap, c->scalar_bytes * N));
+    if (!c->snap_valid) {
+        // first call after mplb_init: take the snapshot, no change to report yet
+        WCHK(c, hipMemcpyAsync(c->rho_snap, c->rho, c->scalar_bytes * N, hipMemcpyDeviceToDevice, c->stream));
+        WCHK(c, hipStreamSynchronize(c->stream));
+        c->snap_valid = true;
+        *max_abs = NAN;
+        return 0;
+    }
+    unsigned long long* part = nullptr;
+    WCHK(c, hipMalloc(&part, sizeof(unsigned long long) * nb));
+    if (c->scalar_bytes == 8) k_change<double><<<nb, kBlock, 0, c->stream>>>((double*)c->rho, (double*)c->rho_snap, N, part);
+    else k_change<float><<<nb, kBlock, 0, c->stream>>>((float*)c->rho, (float*)c->rho_snap, N, part);
+    WCHK(c, hipGetLastError());
+    std::string hostbuf(sizeof(unsigned long long) * nb, '\0');
+    unsigned long long* hp = (unsigned long long*)&hostbuf[0];
+    WCHK(c, hipMemcpyAsync(hp, part, sizeof(unsigned long long) * nb, hipMemcpyDeviceToHost, c->stream));
+    WCHK(c, hipStreamSynchronize(c->stream));
+    WCHK(c, hipFree(part));
+    unsigned long long m = 0;
+    for (unsigned b = 0; b < nb; b++) m = hp[b] > m ? hp[b] : m;
+    double d;
+    std::memcpy(&d, &m, sizeof(d));
+    *max_abs = d;
+    return 0;
+}
+
+}  // extern "C"
