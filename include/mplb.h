@@ -127,6 +127,22 @@ int mplb_lattice_tables(int lbmodel, int* vel, int* rev, double* omega, double* 
  * (fixedRandomT.c:37,81): writes n successive values after srand(seed). */
 int mplb_glibc_rand(unsigned seed, int n, int32_t* out);
 
+/* Files in the reference's formats.  A case directory holding processor0/lattice/*
+ * (written by latticeBox2D/3D + latticeMeshPartition for one rank) is opened from those
+ * tables, as the solver's readLatticeMesh.c does; otherwise the box is built from
+ * properties/latticeProperties. */
+/* processor0/lattice/{points,ghosts,neighbours,recvGhosts,sendGhosts,boundary,vtkCells}
+ * under dir (IO/writeLatticeMesh.c:20-226, one rank). */
+int mplb_write_lattice(const mplb_case* c, const char* dir);
+/* rho, T, p, U, f, g of the current time level into dir/processor0/<time>/ in the case's
+ * dataFormat, asciiRaw or binaryRaw (multiPhasePP.c:291-301 with writeScalarField.c /
+ * writeVectorField.c, including their NaN check: MPLB_E_NAN). */
+int mplb_write_fields(mplb_case* c, const char* dir);
+/* dir/processor0/<time>/<name>, comps values per node, in the case's dataFormat; values
+ * pass through float as the reference's readers do (readScalarFromAsciiRaw.c). */
+int mplb_read_field(const mplb_case* c, const char* dir, int time, const char* name, int comps, double* out,
+                    size_t n);
+
 /* ---------------------------------------------------------------------------------
  * Batched ensembles: many independent small cases advanced together on one GPU.
  * Replaces running multiPhasePP (multiPhasePP.c:157-241) once per case: each case

@@ -13,7 +13,9 @@ import numpy as np
 from .casefiles import CaseSpec, write_case  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmplb.so")
+# MPLB_LIBRARY may name another build of the same library (e.g. one compiled with
+# different flags for a comparison); by default the in-tree build is loaded
+LIB_PATH = os.environ.get("MPLB_LIBRARY") or os.path.join(HERE, "libmplb.so")
 
 FP64, FP32, FP16S = 0, 1, 2
 EXT_LIMRT_D3Q15 = 0x1
@@ -57,6 +59,7 @@ EXPORTS = [
     "mplb_batch_init", "mplb_batch_step", "mplb_batch_run_to_steady", "mplb_batch_get_field",
     "mplb_batch_case_steps", "mplb_batch_tune", "mplb_batch_set_launch", "mplb_batch_launch_info",
     "mplb_density_change", "mplb_set_radial_density",
+    "mplb_write_lattice", "mplb_write_fields", "mplb_read_field",
     "mplb_mlp_create", "mplb_mlp_close", "mplb_mlp_last_error", "mplb_mlp_num_params", "mplb_mlp_set_params",
     "mplb_mlp_get_params", "mplb_mlp_forward", "mplb_mlp_gradient", "mplb_mlp_train", "mplb_mlp_tensor",
 ]
@@ -111,6 +114,9 @@ def library():
         "mplb_batch_set_launch": ([vp, i, i], i),
         "mplb_batch_launch_info": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(C.c_int64)], i),
         "mplb_density_change": ([vp, C.POINTER(d)], i),
+        "mplb_write_lattice": ([vp, cp], i),
+        "mplb_write_fields": ([vp, cp], i),
+        "mplb_read_field": ([vp, cp, i, cp, i, vp, C.c_size_t], i),
         "mplb_set_radial_density": ([vp, vp, i, d, d, vp, d, C.POINTER(d)], i),
         "mplb_mlp_create": ([i, vp, i, i, C.POINTER(vp)], i),
         "mplb_mlp_close": ([vp], i),
@@ -232,6 +238,17 @@ class Case:
         done = C.c_int(0)
         self._check(self.L.mplb_step(self.h, n, C.byref(done)))
         return done.value
+
+    def write_lattice(self, d):
+        self._check(self.L.mplb_write_lattice(self.h, str(d).encode()))
+
+    def write_fields(self, d):
+        self._check(self.L.mplb_write_fields(self.h, str(d).encode()))
+
+    def read_field(self, d, time, name, comps):
+        a = np.zeros((self.N, comps)) if comps > 1 else np.zeros(self.N)
+        self._check(self.L.mplb_read_field(self.h, str(d).encode(), time, name.encode(), comps, a.ctypes.data, a.size))
+        return a
 
     def density_change(self):
         """max |rho - rho at the previous call| (NaN on the first call after init)."""

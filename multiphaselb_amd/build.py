@@ -37,6 +37,7 @@ UNITS = [
     ("kernels_batch.hip", ["-O3", "-ffp-contract=off"] + HIP_DEV),
     ("mlp.hip", ["-O3", "-ffp-contract=off"] + HIP_DEV),
     ("warm.hip", ["-O3", "-ffp-contract=off"] + HIP_DEV),
+    ("fieldio.hip", ["-O2", "-ffp-contract=off"] + HIP_DEV),
     ("capi.hip", ["-O2", "-ffp-contract=off"] + HIP_DEV),
     ("dict.cpp", ["-O2", "-ffp-contract=off"]),
     ("mesh.cpp", ["-O2", "-ffp-contract=off"]),

@@ -199,7 +199,10 @@ int parse_case(mplb_case* c) {
         std::vector<double> v;
         if (lp.vector("spot/centre", &v) && v.size() >= 2) { spot_c[0] = v[0]; spot_c[1] = v[1]; }
     }
-    const std::string merr = build_box(c->mesh, spot_r, spot_c);
+    // the reference solver reads the lattice written by latticeBox + latticeMeshPartition
+    // (readLatticeMesh.c); without those files the box is built from latticeProperties
+    const std::string merr =
+        lattice_files_present(c->dir) ? read_lattice(c->mesh, c->dir) : build_box(c->mesh, spot_r, spot_c);
     if (!merr.empty()) return fail(c, MPLB_E_MODEL, merr);
     for (auto& l : c->mesh.bd)
         for (int32_t id : l)

@@ -119,6 +119,20 @@ __device__ inline void pull(const BArgs& a, const BCase& bc, const Pos& p, const
     }
 }
 
+// density of pulled populations: the reference's index order in FP64 (parity), the
+// inversion-symmetric pair form in FP32 (no drift from biased rounding)
+template <typename T>
+__device__ inline T density(const T* v) {
+    if constexpr (sizeof(T) == 8) {
+        T rho = 0;
+#pragma unroll
+        for (int q = 0; q < 9; q++) rho += v[q];
+        return rho;
+    } else {
+        return rho_sym<D2Q9, T>(v);
+    }
+}
+
 // Populations and density for the collision of the next step.  FROM_INIT is the first
 // step: populations f = f_eq(rho, 0) of the initial density (equilibrium.c:17-85 with
 // U = 0), and the collision uses that density and U = 0 as read from the case
@@ -128,10 +142,7 @@ template <typename S, typename T, int MODE>
 __device__ inline T load_state(const BArgs& a, const BCase& bc, const Pos& p, T* v) {
     if (MODE == FROM_PULL) {
         pull<S, T>(a, bc, p, a.fin, v);
-        T rho = 0;
-#pragma unroll
-        for (int q = 0; q < 9; q++) rho += v[q];
-        return rho;
+        return density<T>(v);
     }
     const T rho = ((const T*)a.rho)[p.gid];
     const T U0[3] = {T(0), T(0), T(0)};
@@ -165,15 +176,32 @@ __global__ void k_batch_collide(BArgs a) {
     const T* psi = (const T*)a.psi;
     const T psic = psi[p.gid];
     T F[2] = {T(0.0), T(0.0)};
+    T psin[9];
 #pragma unroll
-    for (int q = 1; q < 9; q++) {
-        const unsigned l = p.X[1 + D2Q9::t.c[q][0]] + p.Y[1 + D2Q9::t.c[q][1]];
-        const T al = T(D2Q9::t.w[q]) * psi[p.off + l];
+    for (int q = 1; q < 9; q++) psin[q] = psi[p.off + p.X[1 + D2Q9::t.c[q][0]] + p.Y[1 + D2Q9::t.c[q][1]]];
+    constexpr Pairs<D2Q9> PR = make_pairs<D2Q9>();
+    if constexpr (sizeof(T) == 8) {
 #pragma unroll
-        for (int d = 0; d < 2; d++) {
-            const int c = D2Q9::t.c[q][d];
-            if (c == 1) F[d] += al;
-            else if (c == -1) F[d] += -al;
+        for (int q = 1; q < 9; q++) {
+            const T al = T(D2Q9::t.w[q]) * psin[q];
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                const int c = D2Q9::t.c[q][d];
+                if (c == 1) F[d] += al;
+                else if (c == -1) F[d] += -al;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < Pairs<D2Q9>::n; k++) {
+            const int q = PR.q[k];
+            const T dpsi = T(D2Q9::t.w[q]) * (psin[q] - psin[PR.r[k]]);
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                const int c = D2Q9::t.c[q][d];
+                if (c == 1) F[d] += dpsi;
+                else if (c == -1) F[d] -= dpsi;
+            }
         }
     }
     const T beta = -T(a.eos.G) * psic;
@@ -186,11 +214,20 @@ __global__ void k_batch_collide(BArgs a) {
 #pragma unroll
         for (int d = 0; d < 2; d++) {
             T lv = 0;
+            if constexpr (sizeof(T) == 8) {
 #pragma unroll
-            for (int q = 0; q < 9; q++) {
-                const int c = D2Q9::t.c[q][d];
-                if (c == 1) lv += v[q];
-                else if (c == -1) lv += -v[q];
+                for (int q = 0; q < 9; q++) {
+                    const int c = D2Q9::t.c[q][d];
+                    if (c == 1) lv += v[q];
+                    else if (c == -1) lv += -v[q];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < Pairs<D2Q9>::n; k++) {
+                    const int c = D2Q9::t.c[PR.q[k]][d];
+                    if (c == 1) lv += v[PR.q[k]] - v[PR.r[k]];
+                    else if (c == -1) lv -= v[PR.q[k]] - v[PR.r[k]];
+                }
             }
             U[d] = (lv + Ft[d] * T(0.5)) / rho;
         }
@@ -206,7 +243,8 @@ __global__ void k_batch_collide(BArgs a) {
     meq[6] = rho * (-U[1]);
     meq[7] = rho * (U[0] * U[0] - U[1] * U[1]);
     meq[8] = rho * U[0] * U[1];
-    moments<D2Q9, T>(v, m);
+    if constexpr (sizeof(T) == 8) moments<D2Q9, T>(v, m);
+    else moments_sym<D2Q9, T>(v, m);
     const T sigma = T(a.sigma);
     const T L1 = T(a.lam[1]), L2 = T(a.lam[2]);
     const T uF = U[0] * Ft[0] + U[1] * Ft[1];
@@ -225,7 +263,8 @@ __global__ void k_batch_collide(BArgs a) {
         const T l = T(a.lam[q]);
         m[q] = m[q] - l * (m[q] - meq[q]) + (T(1) - T(0.5) * l) * Sf[q];
     }
-    populations<D2Q9, T>(m, v);
+    if constexpr (sizeof(T) == 8) populations<D2Q9, T>(m, v);
+    else populations_sym<D2Q9, T>(m, v);
 #pragma unroll
     for (int q = 0; q < 9; q++) st<S, T>(a.fout, q * a.P + p.gid, v[q], shift_of<T>(bc, q));
 }
@@ -242,9 +281,7 @@ __global__ void k_batch_measure(BArgs a, int compare) {
     if (p.valid) {
         T v[9];
         pull<S, T>(a, bc, p, a.fin, v);
-        T rho = 0;
-#pragma unroll
-        for (int q = 0; q < 9; q++) rho += v[q];
+        const T rho = density<T>(v);
         T* r = (T*)a.rho;
         if (compare) d = __float_as_uint(fabsf((float)(rho - r[p.gid])));
         r[p.gid] = rho;
@@ -293,9 +330,7 @@ __global__ void k_batch_rho_psi(BArgs a) {
     if (!p.valid) return;
     T v[9];
     pull<S, T>(a, bc, p, a.fin, v);
-    T rho = 0;
-#pragma unroll
-    for (int q = 0; q < 9; q++) rho += v[q];
+    const T rho = density<T>(v);
     ((T*)a.rho)[p.gid] = rho;
     ((T*)a.psi)[p.gid] = potential<T>(a.eos, rho, T(bc.T));
 }

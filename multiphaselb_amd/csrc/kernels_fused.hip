@@ -1,21 +1,25 @@
 // Fused structured-box path: FP32 arithmetic, FP32 or FP16-deviation population storage.
 //
 // The state between steps is the post-collision population set of the current step.
-// One time step is two node-parallel passes (three with heat transfer):
+// Isothermal D3Q15 boxes whose links are all the periodic wrap take one single-pass
+// kernel per step (k_fused_march: pull-stream, psi in an LDS ring of planes, force,
+// velocity and collision).  Other boxes take two node-parallel passes (three with heat
+// transfer):
 //   K1  pull-stream f (and g) from the post-collision buffers, apply the boundary
 //       conditions of the node in registers, and write what the neighbours' stencils
 //       need: psi = potential(rho, T) (isothermal) or rho and sum g (thermal);
 //   KH  (thermal) heat source from the rho / T / U stencils, new T, psi;
-//   K2  pull again (the same addresses K1 touched, mostly served by the caches), psi
-//       stencil -> interaction force -> velocity, then the MRT collisions of the next
-//       step, written to the other buffer.
+//   K2  pull again (the same addresses K1 touched), psi stencil -> interaction force ->
+//       velocity, then the MRT collisions of the next step, written to the other buffer.
 // Interior nodes compute the pull source as a fixed offset; nodes on the box surface
 // read the neighbour table, so the reference's duplicate-node periodic links and wall
-// links are followed exactly.  Blocks are numbered so that each XCD owns a contiguous
-// range of z planes and the planes a block pulls from share that XCD's L2.
+// links are followed exactly.  Block b of K1/K2 covers the 256 consecutive node ids
+// [256 b, 256 b + 256).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <unordered_map>
 #include <vector>
@@ -71,6 +75,9 @@ struct FArgs {
     float* out_Ft;
     int first_call;                             // liTempMRT heat source S_old = 0
     float fshift[15], gshift[15];               // FP16 deviation storage shifts
+    // single-pass kernel: column tiles in x, tiles per plane, work items, items per XCD,
+    // planes per chunk
+    int mtx, mtiles, mitems, mper, mzc;
 };
 
 template <typename S>
@@ -239,14 +246,10 @@ __global__ void __launch_bounds__(256) k_fused_density(FArgs a) {
     if (!p.valid) return;
     float v[L::Q];
     pull_f<L, S>(a, p, v);
-    float rho = 0.f;
-#pragma unroll
-    for (int q = 0; q < L::Q; q++) rho += v[q];
+    const float rho = rho_sym<L, float>(v);
     if (THERMAL) {
         pull_g<L, S>(a, p, v);
-        float sg = 0.f;
-#pragma unroll
-        for (int q = 0; q < L::Q; q++) sg += v[q];
+        const float sg = rho_sym<L, float>(v);
         a.rho[p.id] = rho;
         a.sumg[p.id] = sg;
     } else {
@@ -317,68 +320,58 @@ __global__ void __launch_bounds__(256) k_fused_heat(FArgs a) {
     a.psi[id] = potential<float>(a.eos, rho, T);
 }
 
-// K2
-template <class L, typename S, bool THERMAL>
-__global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
-    const NodePos p = node_pos<L>(a);
-    if (!p.valid) return;
-    const int64_t id = p.id, N = a.N;
-    float v[L::Q];
-    pull_f<L, S>(a, p, v);
-    float rho = 0.f;
-#pragma unroll
-    for (int q = 0; q < L::Q; q++) rho += v[q];
-    // interaction force (interactionForce.c:47-79)
-    const float psic = a.psi[id];
-    const bool interior = !p.table || a.onbnd[id] == 0;
-    float Fi[3] = {0.f, 0.f, 0.f};
-    float psin[L::Q];
-    psin[0] = psic;
-    if (interior || a.surf) {
-#pragma unroll
-        for (int q = 1; q < L::Q; q++) {
-            const int64_t n = downstream<L>(a, p, q);
-            psin[q] = n == -1 ? psic : a.psi[n];
-        }
-    }
+// Interaction force (interactionForce.c:47-79), velocity (liMRTVelocity.c /
+// xuMRTVelocity.c) and the MRT collision of f (liMRTCollision.c:33-106 /
+// xuMRTCollision.c:42-92, surface tension terms of liMRTSurfaceTension.c /
+// xuMRTSurfaceTension.c) of one node: v holds its pulled populations on entry and the
+// post-collision populations on exit; psin[q] is psi at the downstream neighbour
+// x + c_q (psin[0] the node's own psi).  interior: the node has no missing link
+// (isOnBnd == 0); full: surface tension applies (no missing link).
+template <class L>
+__device__ inline void collide_node(const FArgs& a, float* v, float rho, const float* psin, bool interior, bool full,
+                                    float* U, float* Fi, float* Ft) {
+    const float psic = psin[0];
+    Fi[0] = Fi[1] = Fi[2] = 0.f;
+    constexpr Pairs<L> P = make_pairs<L>();
     if (interior) {
+        // sum over q of w_q psi(x + c_q) c_q, as pair differences (odd under inversion)
         float F[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-        for (int q = 1; q < L::Q; q++) {
-            const float al = (float)L::t.w[q] * psin[q];
+        for (int k = 0; k < Pairs<L>::n; k++) {
+            const int q = P.q[k];
+            const float dpsi = (float)L::t.w[q] * (psin[q] - psin[P.r[k]]);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const int c = L::t.c[q][d];
-                if (c == 1) F[d] += al;
-                else if (c == -1) F[d] -= al;
+                if (c == 1) F[d] += dpsi;
+                else if (c == -1) F[d] -= dpsi;
             }
         }
         const float beta = -(float)a.eos.G * psic;
 #pragma unroll
         for (int d = 0; d < 3; d++) Fi[d] = F[d] * beta;
     }
-    float Ft[3], Fu[3];
+    float Fu[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const float grav = (rho - a.rho0) * (float)a.eos.g[d];
         Ft[d] = Fi[d] + grav + (interior ? (float)a.eos.fext[d] : 0.f);
         Fu[d] = a.fmodel == 0 ? Ft[d] : Fi[d] + grav;
     }
-    float U[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         float j = 0.f;
 #pragma unroll
-        for (int q = 0; q < L::Q; q++) {
-            const int c = L::t.c[q][d];
-            if (c == 1) j += v[q];
-            else if (c == -1) j -= v[q];
+        for (int k = 0; k < Pairs<L>::n; k++) {
+            const int c = L::t.c[P.q[k]][d];
+            if (c == 1) j += v[P.q[k]] - v[P.r[k]];
+            else if (c == -1) j -= v[P.q[k]] - v[P.r[k]];
         }
         U[d] = (j + 0.5f * Fu[d]) / rho;
     }
     // MRT collision of f (liMRTCollision.c:33-106 / xuMRTCollision.c:42-92)
     float m[L::Q], meq[L::Q], Sf[L::Q];
-    moments<L, float>(v, m);
+    moments_sym<L, float>(v, m);
     const float umag = U[0] * U[0] + U[1] * U[1] + U[2] * U[2];
     const float psi2 = psic * psic;
     if (L::D == 2) {
@@ -440,15 +433,12 @@ __global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
     for (int q = 0; q < L::Q; q++) m[q] = m[q] - a.fl[q] * (m[q] - meq[q]) + (1.f - 0.5f * a.fl[q]) * Sf[q];
     if (a.surf) {
         // liMRTSurfaceTension.c / xuMRTSurfaceTension.c (zero next to missing links)
-        bool full = true;
-#pragma unroll
-        for (int q = 1; q < L::Q; q++)
-            if (p.table && a.nb[q * N + id] == -1) full = false;
         if (full) {
             float Qt[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
-            for (int q = 1; q < L::Q; q++) {
-                const float al = (float)L::t.w[q] * (psin[q] - psic);
+            for (int k = 0; k < Pairs<L>::n; k++) {
+                const int q = P.q[k];
+                const float al = (float)L::t.w[q] * ((psin[q] - psic) + (psin[P.r[k]] - psic));
 #pragma unroll
                 for (int r = 0; r < L::D; r++)
 #pragma unroll
@@ -472,7 +462,37 @@ __global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
             }
         }
     }
-    populations<L, float>(m, v);
+    populations_sym<L, float>(m, v);
+}
+
+// K2
+template <class L, typename S, bool THERMAL>
+__global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
+    const NodePos p = node_pos<L>(a);
+    if (!p.valid) return;
+    const int64_t id = p.id, N = a.N;
+    float v[L::Q];
+    pull_f<L, S>(a, p, v);
+    const float rho = rho_sym<L, float>(v);
+    const float psic = a.psi[id];
+    const bool interior = !p.table || a.onbnd[id] == 0;
+    float psin[L::Q];
+    psin[0] = psic;
+    if (interior || a.surf) {
+#pragma unroll
+        for (int q = 1; q < L::Q; q++) {
+            const int64_t n = downstream<L>(a, p, q);
+            psin[q] = n == -1 ? psic : a.psi[n];
+        }
+    }
+    bool full = true;
+    if (a.surf && p.table) {
+#pragma unroll
+        for (int q = 1; q < L::Q; q++)
+            if (a.nb[q * N + id] == -1) full = false;
+    }
+    float U[3], Fi[3], Ft[3];
+    collide_node<L>(a, v, rho, psin, interior, full, U, Fi, Ft);
 #pragma unroll
     for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, v[q], a.fshift[q]);
 
@@ -481,7 +501,7 @@ __global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
         const float T = a.Tnew[id];
         const float Sv = a.S[id];
         float n[L::Q], neq[L::Q];
-        moments<L, float>(v, n);
+        moments_sym<L, float>(v, n);
 #pragma unroll
         for (int q = 0; q < L::Q; q++) neq[q] = 0.f;
         neq[0] = T;
@@ -525,7 +545,7 @@ __global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
             n[3] += (1.f - a.gl[3] / 2.f) * a.gl[4] * n4;
             n[5] += (1.f - a.gl[5] / 2.f) * a.gl[6] * n6;
         }
-        populations<L, float>(n, v);
+        populations_sym<L, float>(n, v);
 #pragma unroll
         for (int q = 0; q < L::Q; q++) st<S>(a.gout, q * a.P + id, v[q], a.gshift[q]);
         a.Unew[id] = U[0];
@@ -543,6 +563,127 @@ __global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
         for (int d = 0; d < 3; d++) {
             a.out_Fi[d * N + id] = Fi[d];
             a.out_Ft[d * N + id] = Ft[d];
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Single-pass isothermal step on fully periodic D3Q15 boxes.  A workgroup owns a
+// MX x MY column of (x, y) sites over a chunk of z planes and walks up the chunk: at
+// plane z it pulls the populations of plane z + 1 for its sites and for the one-site
+// ring around them, turns them into psi in an LDS ring of four planes, then collides
+// plane z from the populations it pulled one plane earlier (kept in registers) and the
+// psi stencil read from LDS.  psi never goes to memory, and per node the step reads
+// the populations once and writes them once; the ring re-pulls hit the L2, because the
+// workgroups that own those sites run on the same XCD at the same time (work items are
+// handed out so that each XCD takes a contiguous range of chunk-major items).
+constexpr int MX = 32, MY = 8, MHX = MX + 2, MHY = MY + 2, MSLOT = MHX * MHY;
+constexpr int MRING = 2 * MHX + 2 * MY;
+
+__device__ inline int wrapd(int u, int n) { return u < 0 ? u + (n - 1) : (u > n - 1 ? u - (n - 1) : u); }
+
+// populations of node (x, y, z) pulled from x - c_q (duplicate-node periodic wrap) and
+// their sum
+template <class L, typename S>
+__device__ inline float march_pull(const FArgs& a, int x, int y, int z, float* v) {
+    const unsigned nx = (unsigned)a.nx, nxy = (unsigned)a.nxy;
+    const unsigned X[3] = {(unsigned)wrapd(x - 1, a.nx), (unsigned)x, (unsigned)wrapd(x + 1, a.nx)};
+    const unsigned Y[3] = {(unsigned)wrapd(y - 1, a.ny) * nx, (unsigned)y * nx, (unsigned)wrapd(y + 1, a.ny) * nx};
+    const unsigned Z[3] = {(unsigned)wrapd(z - 1, a.nz) * nxy, (unsigned)z * nxy, (unsigned)wrapd(z + 1, a.nz) * nxy};
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) {
+        const void* plane = (const char*)a.fin + (size_t)q * a.P * sizeof(S);
+        v[q] = ld<S>(plane, X[1 - L::t.c[q][0]] + Y[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]], a.fshift[q]);
+    }
+    return rho_sym<L, float>(v);
+}
+
+template <class L, typename S>
+__device__ inline float march_psi(const FArgs& a, int x, int y, int z, float* v) {
+    const float rho = march_pull<L, S>(a, x, y, z, v);
+    const float T = a.Tuniform ? a.Tconst : a.Told[(int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy];
+    return potential<float>(a.eos, rho, T);
+}
+
+// KEEP: the populations pulled for psi at plane z + 1 stay in registers until plane
+// z + 1 collides; otherwise the collision pulls them again (L1/L2 hits), which frees
+// registers for more waves per SIMD.
+template <class L, typename S, bool OUT, bool KEEP, int WAVES>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) k_fused_march(FArgs a) {
+    __shared__ float ring[4][MSLOT];
+    const unsigned wi = (blockIdx.x & 7u) * (unsigned)a.mper + (blockIdx.x >> 3);
+    if (wi >= (unsigned)a.mitems) return;
+    const int chunk = (int)(wi / (unsigned)a.mtiles), tile = (int)(wi - (unsigned)chunk * (unsigned)a.mtiles);
+    const int x0 = (tile % a.mtx) * MX, y0 = (tile / a.mtx) * MY;
+    const int z0 = chunk * a.mzc, z1 = min(z0 + a.mzc, a.nz);
+    const int t = threadIdx.x, tx = t % MX, ty = t / MX;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool owner = x < a.nx && y < a.ny;
+    const bool edge = x <= a.nx && y <= a.ny;          // raw n wraps onto site 1
+    const int ex = wrapd(x, a.nx), ey = wrapd(y, a.ny);
+    // this thread's site on the ring around the column (threads < MRING)
+    int rhx = 0, rhy = 0;
+    if (t < MHX) {
+        rhx = t;
+    } else if (t < 2 * MHX) {
+        rhx = t - MHX;
+        rhy = MHY - 1;
+    } else if (t < 2 * MHX + MY) {
+        rhy = 1 + (t - 2 * MHX);
+    } else {
+        rhx = MHX - 1;
+        rhy = 1 + (t - 2 * MHX - MY);
+    }
+    const bool ring_site = t < MRING && x0 + rhx - 1 <= a.nx && y0 + rhy - 1 <= a.ny;
+    const int rx = wrapd(x0 + rhx - 1, a.nx), ry = wrapd(y0 + rhy - 1, a.ny);
+    float fc[L::Q], fn[L::Q], tmp[L::Q];
+    // psi of raw plane zr (z0 - 1 <= zr <= z1) into its ring slot; own populations -> fn
+    auto plane = [&](int zr) {
+        const int zw = wrapd(zr, a.nz);
+        float* sl = ring[(zr - z0 + 1) & 3];
+        if (edge) sl[(ty + 1) * MHX + tx + 1] = march_psi<L, S>(a, ex, ey, zw, KEEP ? fn : tmp);
+        if (ring_site) sl[rhy * MHX + rhx] = march_psi<L, S>(a, rx, ry, zw, tmp);
+    };
+    plane(z0 - 1);
+    plane(z0);
+    if (KEEP) {
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
+    }
+    for (int z = z0; z < z1; z++) {
+        plane(z + 1);
+        __syncthreads();
+        if (owner) {
+            float rho = 0.f;
+            if (KEEP) {
+                rho = rho_sym<L, float>(fc);
+            } else {
+                rho = march_pull<L, S>(a, x, y, z, fc);
+            }
+            float psin[L::Q];
+#pragma unroll
+            for (int q = 0; q < L::Q; q++)
+                psin[q] = ring[(z + L::t.c[q][2] - z0 + 1) & 3][(ty + 1 + L::t.c[q][1]) * MHX + tx + 1 + L::t.c[q][0]];
+            float U[3], Fi[3], Ft[3];
+            collide_node<L>(a, fc, rho, psin, true, true, U, Fi, Ft);
+            const int64_t id = (int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy;
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, fc[q], a.fshift[q]);
+            if (OUT) {
+                const int64_t N = a.N;
+                a.out_rho[id] = rho;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    a.out_U[d * N + id] = U[d];
+                    a.out_Fi[d * N + id] = Fi[d];
+                    a.out_Ft[d * N + id] = Ft[d];
+                }
+            }
+        }
+        if (KEEP) {
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
         }
     }
 }
@@ -594,6 +735,10 @@ struct Fused {
     int64_t P = 0;                 // population plane pitch
     int face_table = 0;            // faces whose links differ from the periodic wrap
     float fshift[15] = {0}, gshift[15] = {0};
+    bool march = false;            // single-pass kernel (isothermal, fully periodic D3Q15)
+    bool march_keep = true;        // keep pulled populations in registers between planes
+    int march_waves = 4;           // register budget: waves per SIMD
+    int mtx = 0, mtiles = 0, mitems = 0, mper = 0, mzc = 0;
 };
 
 Fused* F(mplb_case* c) { return (Fused*)c->fz; }
@@ -663,9 +808,27 @@ FArgs make_args(mplb_case* c) {
     a.Uold = z->thermal ? z->U[z->cur] : (const float*)c->U;
     a.Unew = z->thermal ? z->U[1 - z->cur] : nullptr;
     a.Tuniform = 0;
+    a.mtx = z->mtx;
+    a.mtiles = z->mtiles;
+    a.mitems = z->mitems;
+    a.mper = z->mper;
+    a.mzc = z->mzc;
     std::memcpy(a.fshift, z->fshift, sizeof(a.fshift));
     std::memcpy(a.gshift, z->gshift, sizeof(a.gshift));
     return a;
+}
+
+template <class L, typename S, bool OUT>
+void launch_march(mplb_case* c, const FArgs& a) {
+    const dim3 grid(8u * (unsigned)a.mper);
+    const Fused* z = F(c);
+    if (z->march_waves == 3) {
+        if (z->march_keep) k_fused_march<L, S, OUT, true, 3><<<grid, 256, 0, c->stream>>>(a);
+        else k_fused_march<L, S, OUT, false, 3><<<grid, 256, 0, c->stream>>>(a);
+    } else {
+        if (z->march_keep) k_fused_march<L, S, OUT, true, 4><<<grid, 256, 0, c->stream>>>(a);
+        else k_fused_march<L, S, OUT, false, 4><<<grid, 256, 0, c->stream>>>(a);
+    }
 }
 
 template <class L, typename S>
@@ -681,6 +844,8 @@ void launch_step(mplb_case* c, FArgs a, bool thermal, bool outputs) {
         k_fused_density<L, S, true><<<grid, block, 0, c->stream>>>(a);
         k_fused_heat<L><<<grid, block, 0, c->stream>>>(a);
         k_fused_collide<L, S, true><<<grid, block, 0, c->stream>>>(a);
+    } else if (F(c)->march) {
+        if constexpr (L::D == 3) launch_march<L, S, false>(c, a);
     } else {
         k_fused_density<L, S, false><<<grid, block, 0, c->stream>>>(a);
         k_fused_collide<L, S, false><<<grid, block, 0, c->stream>>>(a);
@@ -728,6 +893,9 @@ void launch_step_timed(mplb_case* c, FArgs a, bool thermal, bool outputs) {
         }
         Ev e(c, 6);
         k_fused_collide<L, S, true><<<grid, block, 0, c->stream>>>(a);
+    } else if (F(c)->march) {
+        Ev e(c, 6);
+        if constexpr (L::D == 3) launch_march<L, S, false>(c, a);
     } else {
         {
             Ev e(c, 7);
@@ -840,8 +1008,14 @@ int materialize(mplb_case* c) {
     a.out_Fi = (float*)c->Fi;
     a.out_Ft = (float*)c->Ft;
     a.out_U = (float*)c->U;
-    if (z->thermal) k_fused_collide<L, S, true><<<grid, block, 0, c->stream>>>(a);
-    else k_fused_collide<L, S, false><<<grid, block, 0, c->stream>>>(a);
+    if (z->thermal) {
+        k_fused_collide<L, S, true><<<grid, block, 0, c->stream>>>(a);
+    } else if (z->march) {
+        // the step's own kernel with outputs: it rewrites fa with the same values
+        if constexpr (L::D == 3) launch_march<L, S, true>(c, a);
+    } else {
+        k_fused_collide<L, S, false><<<grid, block, 0, c->stream>>>(a);
+    }
     if (z->thermal) {
         const int64_t N = c->mesh.N;
         FCHK(c, hipMemcpyAsync(c->temp, z->T[z->cur], sizeof(float) * N, hipMemcpyDeviceToDevice, c->stream));
@@ -871,6 +1045,7 @@ bool fused_supported(const mplb_case* c, std::string* why) {
     };
     if (c->opts.precision == MPLB_FP64) return no("the fused kernels compute in FP32");
     const Mesh& m = c->mesh;
+    if (!m.structured) return no("lattice nodes not in structured box order");
     if (m.nz > 1 && m.d == 2) return no("2D box with nz > 1");
     const int fm = c->fc[0].mp.model;
     if (!(fm == M_LIMRT && m.lbmodel == 0) && !(fm == M_XUMRT && m.lbmodel == 1)) return no("f model/lattice");
@@ -988,6 +1163,30 @@ int fused_create(mplb_case* c) {
             if (differs) mask |= faces;
         }
         z->face_table = mask;
+    }
+    // single-pass kernel: isothermal D3Q15 with every link the periodic wrap and no
+    // boundary node (MPLB_FUSED_KERNEL=twopass keeps the two-pass kernels)
+    {
+        const char* env = std::getenv("MPLB_FUSED_KERNEL");
+        const bool twopass = env && std::strcmp(env, "twopass") == 0;
+        bool any_bnd = false;
+        for (int64_t id = 0; id < N && !any_bnd; id++) any_bnd = m.onbnd[id] != 0;
+        z->march = !twopass && m.lbmodel == 1 && !z->thermal && z->face_table == 0 && !any_bnd && m.nx >= 3 &&
+                   m.ny >= 3 && m.nz >= 3;
+        if (z->march) {
+            const char* rp = std::getenv("MPLB_MARCH_REPULL");
+            z->march_keep = !(rp && rp[0] == '1');
+            const char* mw = std::getenv("MPLB_MARCH_WAVES");
+            z->march_waves = (mw && mw[0] == '3') ? 3 : 4;
+            z->mtx = (m.nx + MX - 1) / MX;
+            z->mtiles = z->mtx * ((m.ny + MY - 1) / MY);
+            int nchunks = (2048 + z->mtiles - 1) / z->mtiles;
+            nchunks = std::max(1, std::min(nchunks, m.nz / 8));
+            z->mzc = (m.nz + nchunks - 1) / nchunks;
+            nchunks = (m.nz + z->mzc - 1) / z->mzc;
+            z->mitems = z->mtiles * nchunks;
+            z->mper = (z->mitems + 7) / 8;
+        }
     }
     FCHK(c, hipMalloc(&z->bcf, N));
     FCHK(c, hipMemcpy(z->bcf, bcf.data(), N, hipMemcpyHostToDevice));

@@ -20,12 +20,18 @@ struct Mesh {
     std::vector<uint8_t> onbnd;              // isOnBnd
     std::vector<std::string> bdnames;
     std::vector<std::vector<int32_t>> bd;    // boundary lists in file order
+    std::vector<int32_t> points;             // [N][3] when read from lattice files (else implied)
+    bool structured = true;                  // node id = i + j nx (+ k nx ny)
 };
 
 // Builds the neighbour table and boundary lists of an nx x ny (x nz) box the way
 // latticeBox2D/latticeBox3D followed by a one-rank latticeMeshPartition do.  Returns
 // an empty string on success or the reference's error text.
 std::string build_box(Mesh& m, double spot_radius, const double spot_centre[2]);
+
+// processor0/lattice/* of a one-rank case (fieldio.hip)
+bool lattice_files_present(const std::string& dir);
+std::string read_lattice(Mesh& m, const std::string& dir);
 
 // The wrap-around index the fused kernels use for a periodic axis with duplicate end
 // nodes: coordinate u in [-1, n] maps to u in [0, n-1] with period n-1.

@@ -150,4 +150,127 @@ __device__ __host__ inline void litemp_geq(T temp, const T U[3], T* geq) {
     populations<L, T>(n, geq);
 }
 
+// ---------------------------------------------------------------------------
+// Inversion-symmetric forms for the reduced-precision kernels.  Summing populations
+// in index order is not an odd function of the state under x -> -x (which swaps f_q and
+// f_rev(q)): fl(fl(a - b) + c) and fl(fl(b - a) - c) need not be exact negatives, so a
+// symmetric droplet picks up a bias of a fraction of an ulp per node per step and its
+// total momentum drifts linearly in FP32 (in FP64 the same bias is ~1e-16 and harmless).
+// Here every sum is formed from pair sums s = f_q + f_rev and differences
+// d = f_q - f_rev: even moments from s, odd moments from d, and populations as
+// f_q = E + O, f_rev = E - O, so the arithmetic is exactly equivariant under inversion
+// and the rounding errors of mirror nodes cancel instead of adding up.
+template <class L>
+struct Pairs {
+    static constexpr int n = (L::Q - 1) / 2;
+    int q[n], r[n];
+};
+
+template <class L>
+constexpr Pairs<L> make_pairs() {
+    Pairs<L> p{};
+    int k = 0;
+    for (int q = 1; q < L::Q; q++)
+        if (q < L::t.rev[q]) {
+            p.q[k] = q;
+            p.r[k] = L::t.rev[q];
+            k++;
+        }
+    return p;
+}
+
+// +1: moment row j is even under inversion (M[j][q] == M[j][rev q]), -1: odd
+template <class L>
+constexpr int moment_parity(int j) {
+    const Pairs<L> P = make_pairs<L>();
+    for (int k = 0; k < Pairs<L>::n; k++) {
+        const int a = L::t.M[j][P.q[k]];
+        if (a != 0) return a == L::t.M[j][P.r[k]] ? 1 : -1;
+    }
+    return 1;
+}
+
+// every row is either even or odd, and odd rows do not see the rest population
+template <class L>
+constexpr bool parity_consistent() {
+    const Pairs<L> P = make_pairs<L>();
+    for (int j = 0; j < L::Q; j++) {
+        const int s = moment_parity<L>(j);
+        if (s == -1 && L::t.M[j][0] != 0) return false;
+        for (int k = 0; k < Pairs<L>::n; k++)
+            if (L::t.M[j][P.q[k]] != s * L::t.M[j][P.r[k]]) return false;
+    }
+    return true;
+}
+
+template <typename T>
+__device__ __host__ inline T imul(int c, T x) {
+    return c == 1 ? x : (c == -1 ? -x : T(c) * x);
+}
+
+// rho = f_0 + sum of pair sums
+template <class L, typename T>
+__device__ __host__ inline T rho_sym(const T* v) {
+    constexpr Pairs<L> P = make_pairs<L>();
+    T r = v[0];
+#pragma unroll
+    for (int k = 0; k < Pairs<L>::n; k++) r += v[P.q[k]] + v[P.r[k]];
+    return r;
+}
+
+// m = M v from pair sums (even rows) and pair differences (odd rows)
+template <class L, typename T>
+__device__ __host__ inline void moments_sym(const T* v, T* m) {
+    constexpr Pairs<L> P = make_pairs<L>();
+    constexpr int NP = Pairs<L>::n;
+    T sp[NP], dp[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        sp[k] = v[P.q[k]] + v[P.r[k]];
+        dp[k] = v[P.q[k]] - v[P.r[k]];
+    }
+#pragma unroll
+    for (int j = 0; j < L::Q; j++) {
+        T r = T(0);
+        if (moment_parity<L>(j) == 1) {
+            if (L::t.M[j][0] != 0) r = imul<T>(L::t.M[j][0], v[0]);
+#pragma unroll
+            for (int k = 0; k < NP; k++)
+                if (L::t.M[j][P.q[k]] != 0) r += imul<T>(L::t.M[j][P.q[k]], sp[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NP; k++)
+                if (L::t.M[j][P.q[k]] != 0) r += imul<T>(L::t.M[j][P.q[k]], dp[k]);
+        }
+        m[j] = r;
+    }
+}
+
+// v = M^-1 m as even part E and odd part O per pair: f_q = E + O, f_rev = E - O
+template <class L, typename T>
+__device__ __host__ inline void populations_sym(const T* m, T* v) {
+    constexpr Pairs<L> P = make_pairs<L>();
+    T r0 = T(0);
+#pragma unroll
+    for (int j = 0; j < L::Q; j++)
+        if (moment_parity<L>(j) == 1 && L::t.invM[0][j] != 0.0) r0 += T(L::t.invM[0][j]) * m[j];
+    v[0] = r0;
+#pragma unroll
+    for (int k = 0; k < Pairs<L>::n; k++) {
+        const int q = P.q[k];
+        T E = T(0), O = T(0);
+#pragma unroll
+        for (int j = 0; j < L::Q; j++) {
+            const double c = L::t.invM[q][j];
+            if (c == 0.0) continue;
+            if (moment_parity<L>(j) == 1) E += T(c) * m[j];
+            else O += T(c) * m[j];
+        }
+        v[q] = E + O;
+        v[P.r[k]] = E - O;
+    }
+}
+
+static_assert(parity_consistent<D2Q9>() && parity_consistent<D3Q15>(), "moment rows must be even or odd");
+
 }  // namespace mplb

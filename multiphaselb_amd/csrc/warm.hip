@@ -105,6 +105,7 @@ int mplb_set_radial_density(mplb_case* c, const double* prof, int nprof, double 
                             double R, double* mass) {
     if (!c || !prof || nprof < 2 || !(dd > 0) || !centre) return MPLB_E_ARG;
     if (c->opts.device < 0 || !c->rho) return wfail(c, MPLB_E_STATE, "the case has no device fields (device < 0)");
+    if (!c->mesh.structured) return wfail(c, MPLB_E_MODEL, "radial fields need a lattice in structured box order");
     WCHK(c, hipSetDevice(c->opts.device));
     const int64_t N = c->mesh.N;
     const unsigned nb = (unsigned)((N + kBlock - 1) / kBlock);
