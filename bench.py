@@ -111,18 +111,24 @@ def cpu_baseline(args, budget_s=12.0):
                        ("xuMRT+myMRT thermal" if args.thermal else "xuMRT", steps, dt))
 
 
-def laplace(rho, p, L):
-    """Interior/exterior density and pressure (2x2 nodes around the box centre, the box
-    corner), the equivalent radius from the mass on the L-1 periodic sites, and the
-    Laplace product dp * R of one steady case."""
-    r = rho.reshape(L, L)[: L - 1, : L - 1]
-    pp = p.reshape(L, L)
-    c = (L - 1) // 2
-    rin = float(rho.reshape(L, L)[c:c + 2, c:c + 2].mean())
-    pin = float(pp[c:c + 2, c:c + 2].mean())
-    rout, pout = float(r[0, 0]), float(pp[0, 0])
-    area = (float(r.sum()) - rout * (L - 1) ** 2) / (rin - rout)
+def laplace(rho, p, L, rho_v, rho_l):
+    """Interior/exterior density and pressure (averages over the disc r < R/2 around the
+    box centre and over the corner region r > r_max - 3), the equivalent radius from
+    the mass on the L-1 periodic sites, and the Laplace product dp * R of one steady
+    case; None when the droplet or bubble has dissolved."""
+    ii, jj = np.meshgrid(np.arange(L), np.arange(L), indexing="xy")
+    c = (L - 1) / 2.0
+    r = np.sqrt((ii - c) ** 2 + (jj - c) ** 2).ravel()
+    uniq = ((ii < L - 1) & (jj < L - 1)).ravel()
+    outer = r > r.max() - 3.0
+    rout, pout = float(rho[outer].mean()), float(p[outer].mean())
+    core = r < 2.0
+    if abs(float(rho[core].mean()) - rout) < 0.25 * (rho_l - rho_v):
+        return None
+    area = (float(rho[uniq].sum()) - rout * (L - 1) ** 2) / (float(rho[core].mean()) - rout)
     R = float(np.sqrt(max(area, 0.0) / np.pi))
+    inner = r < max(1.5, 0.5 * R)
+    rin, pin = float(rho[inner].mean()), float(p[inner].mean())
     return dict(rho_in=rin, rho_out=rout, dp=pin - pout, R=R, gamma=(pin - pout) * R)
 
 
@@ -157,10 +163,11 @@ def run_ensemble(args):
         t1 = time.perf_counter()
         nconv, steps, delta, state = b.run_to_steady(thr[prec], check_every=100, max_steps=args.max_steps)
         t2 = time.perf_counter()
-        lap = [laplace(b.get(k, "rho"), b.get(k, "p"), c["L"]) if state[k] == 1 else None
+        lap = [laplace(b.get(k, "rho"), b.get(k, "p"), c["L"], c["rho_v"], c["rho_l"]) if state[k] == 1 else None
                for k, c in enumerate(ens)]
         lu = float(sum(c["L"] ** 2 * int(steps[k]) for k, c in enumerate(ens)))
         res[prec] = dict(steady=int(nconv), diverged=int((state == 2).sum()), threshold=thr[prec],
+                         dissolved=int(sum(1 for k in range(len(ens)) if state[k] == 1 and lap[k] is None)),
                          wall_s=t2 - t0, run_s=t2 - t1, tune_s=t1 - t0, block=block, xcd_order=xcd,
                          steps_median=float(np.median(steps)), steps_max=int(steps.max()),
                          mlups=lu / (t2 - t0) / 1e6, lap=lap, state=state)
@@ -194,6 +201,7 @@ def run_ensemble(args):
                          "rel_err_vs_fp64": float(abs(s - sr) / abs(sr))})
         out_prec[prec] = {
             "cases_per_s": round(r["steady"] / r["wall_s"], 3), "steady": r["steady"], "diverged": r["diverged"],
+            "dissolved": r["dissolved"], "laplace_cases": len(both),
             "threshold": r["threshold"], "wall_s": round(r["wall_s"], 2), "tune_s": round(r["tune_s"], 2),
             "mlups": round(r["mlups"], 1), "steps_median": r["steps_median"], "steps_max": r["steps_max"],
             "launch": {"block": r["block"], "xcd_order": r["xcd_order"]},
@@ -290,9 +298,12 @@ def run_warmstart(args):
                 continue
             L = c["L"]
             r = W.node_radius((L, L), ((L - 1) / 2.0, (L - 1) / 2.0))
-            y, _ = W.extract_profile(b.get(k, "rho"), r, c["rho_v"], c["rho_l"], c["droplet"])
+            try:
+                y, _ = W.extract_profile(b.get(k, "rho"), r, c["rho_v"], c["rho_l"], c["droplet"])
+            except ValueError:
+                continue
             samples_X.append(W.features(c["tr"], c["radius"], L, c["droplet"], 2))
-            samples_Y.append(np.concatenate([y, np.zeros(W.NPROF)]))
+            samples_Y.append(np.concatenate([W.target_from_profile(y, c["droplet"]), np.zeros(W.NPROF)]))
         b.close()
         log("2D: %d/%d steady in %.1f s, steps median %d" % (nconv, len(ens), time.perf_counter() - t_data,
                                                             int(np.median(steps))))
@@ -303,6 +314,7 @@ def run_warmstart(args):
     N = case.N
     cen = ((L3 - 1) / 2.0,) * 3
     r3 = W.node_radius((L3, L3, L3), cen)
+    sites3 = W.distinct_sites((L3, L3, L3))
 
     def cold_field(c):
         inside, outside = (c["rho_l"], c["rho_v"]) if c["droplet"] else (c["rho_v"], c["rho_l"])
@@ -317,19 +329,35 @@ def run_warmstart(args):
         case.init()
 
     t3 = time.perf_counter()
-    train3 = bubble3d_cases(args.ws_3d, 13, L3, 0.5)
+    train3 = bubble3d_cases(args.ws_3d_offset + args.ws_3d, 13, L3, 0.5)[args.ws_3d_offset:]
     for i, c in enumerate(train3):
         start(cold_field(c))
         st, wall, ok = run_to_steady_case(case, c["rho_l"], thr, 100, args.max_steps)
+        y = None
         if ok:
-            y, _ = W.extract_profile(case.get("rho"), r3, c["rho_v"], c["rho_l"], c["droplet"])
+            try:
+                y, _ = W.extract_profile(case.get("rho"), r3, c["rho_v"], c["rho_l"], c["droplet"])
+            except ValueError:
+                y = None
+        if y is not None:
             samples_X.append(W.features(c["tr"], c["radius"], L3, c["droplet"], 3))
-            samples_Y.append(np.concatenate([y, np.zeros(W.NPROF)]))
+            samples_Y.append(np.concatenate([W.target_from_profile(y, c["droplet"]), np.zeros(W.NPROF)]))
         if i % 16 == 0:
             log("3D train case %d: %d steps %.2f s steady %s" % (i, st, wall, ok))
     log("3D training runs: %.1f s" % (time.perf_counter() - t3))
-    X = np.array(samples_X, np.float32)
-    Y = np.array(samples_Y, np.float32)
+    X = np.array(samples_X, np.float32).reshape(-1, W.N_IN)
+    Y = np.array(samples_Y, np.float32).reshape(-1, W.N_OUT)
+    if args.ws_load_data:
+        # profiles from earlier solver runs (same generator, other case indices)
+        for path in args.ws_load_data.split(","):
+            z = np.load(path)
+            X = np.concatenate([X, z["X"]])
+            Y = np.concatenate([Y, z["Y"]])
+    if args.ws_save_data:
+        np.savez(args.ws_save_data, X=X, Y=Y)
+    if args.ws_eval == 0:
+        print(json.dumps({"saved_samples": int(X.shape[0]), "path": args.ws_save_data}))
+        return 0
     t_train = time.perf_counter()
     net = M.Mlp(W.WIDTHS, args.ws_batch)
     net.set_params(W.init_params(W.WIDTHS, 0))
@@ -341,13 +369,13 @@ def run_warmstart(args):
     rows = []
     for i, c in enumerate(evals):
         rho0 = cold_field(c)
-        mass = float(rho0.sum())
+        mass = float(rho0[sites3].sum())
         start(rho0)
         st_c, wall_c, ok_c = run_to_steady_case(case, c["rho_l"], thr, 100, args.max_steps)
         rc = case.get("rho")
         tw = time.perf_counter()
-        y = net.forward(W.features(c["tr"], c["radius"], L3, c["droplet"], 3)[None, :])[0, :W.NPROF]
-        prof = W.profile_density(y, c["rho_v"], c["rho_l"])
+        o = net.forward(W.features(c["tr"], c["radius"], L3, c["droplet"], 3)[None, :])[0, :W.NPROF]
+        prof = W.profile_density(W.profile_from_output(o, c["droplet"]), c["rho_v"], c["rho_l"])
         W.raster_case(case, prof, cen, mass, c["radius"])
         start(None)
         t_prop = time.perf_counter() - tw
@@ -423,6 +451,9 @@ def main():
     ap.add_argument("--ws-size", type=int, default=128)
     ap.add_argument("--ws-iters", type=int, default=4000)
     ap.add_argument("--ws-batch", type=int, default=256)
+    ap.add_argument("--ws-3d-offset", type=int, default=0, help="warmstart: first 3D training case index")
+    ap.add_argument("--ws-save-data", default="", help="warmstart: save the training profiles (npz)")
+    ap.add_argument("--ws-load-data", default="", help="warmstart: add training profiles from npz files")
     ap.add_argument("--cases", type=int, default=1024, help="ensemble: number of cases")
     ap.add_argument("--ens-precisions", default="fp32,fp16s", help="ensemble: precisions besides fp64")
     ap.add_argument("--steady-fp64", type=float, default=1e-9)

@@ -197,8 +197,9 @@ int mplb_batch_launch_info(const mplb_batch* b, int* block, int* xcd_order, int*
 int mplb_density_change(mplb_case* c, double* max_abs);
 /* rho(x) = prof(|x - centre| - R) on the stored lattice (x = node coordinates), prof
  * sampled at d0 + k dd, k = 0..nprof-1, linear interpolation, clamped at both ends;
- * writes the case's density (call mplb_init afterwards) and returns the total mass of
- * the stored field in *mass. */
+ * writes the case's density (call mplb_init afterwards) and returns in *mass the mass of
+ * the distinct sites: on a periodic axis of a built box the last node duplicates the
+ * first (period n - 1) and is left out of the sum. */
 int mplb_set_radial_density(mplb_case* c, const double* prof, int nprof, double d0, double dd, const double* centre,
                             double R, double* mass);
 

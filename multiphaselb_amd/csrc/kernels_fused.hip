@@ -688,6 +688,123 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     }
 }
 
+// Same step with a 32 x TY column and the ring sites spread over groups of G lanes: each
+// lane of a group pulls the pairs k = l, l + G, ... of the site's populations and lane 0
+// adds the pair sums in the order rho_sym uses, so a ring site's psi is bit-identical to
+// the psi its owner computes.  The pull work is balanced over all waves of the block.
+template <class L, typename S, bool OUT, int TY, int G>
+__global__ void __launch_bounds__(32 * TY) __attribute__((amdgpu_waves_per_eu(4))) k_fused_march_b(FArgs a) {
+    constexpr int HX = MX + 2, HY = TY + 2, SLOT = HX * HY, NRING = 2 * HX + 2 * TY;
+    constexpr Pairs<L> P = make_pairs<L>();
+    constexpr int NP = Pairs<L>::n, KPL = (NP + G - 1) / G;
+    static_assert(G * NRING <= 32 * TY && 64 % G == 0, "ring groups must fit the block");
+    __shared__ float ring[4][SLOT];
+    const unsigned wi = (blockIdx.x & 7u) * (unsigned)a.mper + (blockIdx.x >> 3);
+    if (wi >= (unsigned)a.mitems) return;
+    const int chunk = (int)(wi / (unsigned)a.mtiles), tile = (int)(wi - (unsigned)chunk * (unsigned)a.mtiles);
+    const int x0 = (tile % a.mtx) * MX, y0 = (tile / a.mtx) * TY;
+    const int z0 = chunk * a.mzc, z1 = min(z0 + a.mzc, a.nz);
+    const int t = threadIdx.x, tx = t % MX, ty = t / MX;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool owner = x < a.nx && y < a.ny;
+    const bool edge = x <= a.nx && y <= a.ny;
+    const int ex = wrapd(x, a.nx), ey = wrapd(y, a.ny);
+    // ring site of this lane's group
+    const int g = t / G, lane = t % G;
+    int rhx = 0, rhy = 0;
+    if (g < HX) {
+        rhx = g;
+    } else if (g < 2 * HX) {
+        rhx = g - HX;
+        rhy = HY - 1;
+    } else if (g < 2 * HX + TY) {
+        rhy = 1 + (g - 2 * HX);
+    } else {
+        rhx = HX - 1;
+        rhy = 1 + (g - 2 * HX - TY);
+    }
+    const bool ring_site = g < NRING && x0 + rhx - 1 <= a.nx && y0 + rhy - 1 <= a.ny;
+    const int rx = wrapd(x0 + rhx - 1, a.nx), ry = wrapd(y0 + rhy - 1, a.ny);
+    const unsigned nx = (unsigned)a.nx, nxy = (unsigned)a.nxy;
+    const unsigned RX[3] = {(unsigned)wrapd(rx - 1, a.nx), (unsigned)rx, (unsigned)wrapd(rx + 1, a.nx)};
+    const unsigned RY[3] = {(unsigned)wrapd(ry - 1, a.ny) * nx, (unsigned)ry * nx, (unsigned)wrapd(ry + 1, a.ny) * nx};
+    float fc[L::Q], fn[L::Q];
+    auto plane = [&](int zr) {
+        const int zw = wrapd(zr, a.nz);
+        float* sl = ring[(zr - z0 + 1) & 3];
+        if (edge) sl[(ty + 1) * HX + tx + 1] = march_psi<L, S>(a, ex, ey, zw, fn);
+        // ring: pair sums of this lane's pairs
+        const unsigned RZ[3] = {(unsigned)wrapd(zw - 1, a.nz) * nxy, (unsigned)zw * nxy, (unsigned)wrapd(zw + 1, a.nz) * nxy};
+        float part[KPL];
+        float v0 = 0.f;
+#pragma unroll
+        for (int j = 0; j < KPL; j++) part[j] = 0.f;
+        if (ring_site) {
+#pragma unroll
+            for (int j = 0; j < KPL; j++) {
+#pragma unroll
+                for (int ll = 0; ll < G; ll++) {
+                    const int k = j * G + ll;
+                    if (k >= NP || ll != lane) continue;
+                    const int q = P.q[k], r = P.r[k];
+                    const float vq = ld<S>((const char*)a.fin + (size_t)q * a.P * sizeof(S),
+                                           RX[1 - L::t.c[q][0]] + RY[1 - L::t.c[q][1]] + RZ[1 - L::t.c[q][2]],
+                                           a.fshift[q]);
+                    const float vr = ld<S>((const char*)a.fin + (size_t)r * a.P * sizeof(S),
+                                           RX[1 - L::t.c[r][0]] + RY[1 - L::t.c[r][1]] + RZ[1 - L::t.c[r][2]],
+                                           a.fshift[r]);
+                    part[j] = vq + vr;
+                }
+            }
+            if (lane == 0) v0 = ld<S>(a.fin, RX[1] + RY[1] + RZ[1], a.fshift[0]);
+        }
+        // lane 0 adds v0 and the pair sums in pair order (as rho_sym)
+        const int base = (t & 63) - lane;
+        float rho = v0;
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            const float sk = __shfl(part[k / G], base + (k % G), 64);
+            rho += sk;
+        }
+        if (ring_site && lane == 0) {
+            const float T = a.Tuniform ? a.Tconst : a.Told[(int64_t)rx + (int64_t)ry * a.nx + (int64_t)zw * a.nxy];
+            sl[rhy * HX + rhx] = potential<float>(a.eos, rho, T);
+        }
+    };
+    plane(z0 - 1);
+    plane(z0);
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
+    for (int z = z0; z < z1; z++) {
+        plane(z + 1);
+        __syncthreads();
+        if (owner) {
+            const float rho = rho_sym<L, float>(fc);
+            float psin[L::Q];
+#pragma unroll
+            for (int q = 0; q < L::Q; q++)
+                psin[q] = ring[(z + L::t.c[q][2] - z0 + 1) & 3][(ty + 1 + L::t.c[q][1]) * HX + tx + 1 + L::t.c[q][0]];
+            float U[3], Fi[3], Ft[3];
+            collide_node<L>(a, fc, rho, psin, true, true, U, Fi, Ft);
+            const int64_t id = (int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy;
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, fc[q], a.fshift[q]);
+            if (OUT) {
+                const int64_t N = a.N;
+                a.out_rho[id] = rho;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    a.out_U[d * N + id] = U[d];
+                    a.out_Fi[d * N + id] = Fi[d];
+                    a.out_Ft[d * N + id] = Ft[d];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
+    }
+}
+
 // Post-stream, post-boundary populations of the current time level (what the
 // reference holds in f.value after updateBoundaries), FP32 SoA.
 template <class L, typename S, bool G>
@@ -738,6 +855,7 @@ struct Fused {
     bool march = false;            // single-pass kernel (isothermal, fully periodic D3Q15)
     bool march_keep = true;        // keep pulled populations in registers between planes
     int march_waves = 4;           // register budget: waves per SIMD
+    int march_tile = 8;            // 8: 32 x 8 columns; 9: 32 x 8, balanced ring; 16: 32 x 16
     int mtx = 0, mtiles = 0, mitems = 0, mper = 0, mzc = 0;
 };
 
@@ -822,6 +940,14 @@ template <class L, typename S, bool OUT>
 void launch_march(mplb_case* c, const FArgs& a) {
     const dim3 grid(8u * (unsigned)a.mper);
     const Fused* z = F(c);
+    if (z->march_tile == 16) {
+        k_fused_march_b<L, S, OUT, 16, 4><<<grid, 512, 0, c->stream>>>(a);
+        return;
+    }
+    if (z->march_tile == 9) {          // 32 x 8 with balanced ring pulls
+        k_fused_march_b<L, S, OUT, 8, 2><<<grid, 256, 0, c->stream>>>(a);
+        return;
+    }
     if (z->march_waves == 3) {
         if (z->march_keep) k_fused_march<L, S, OUT, true, 3><<<grid, 256, 0, c->stream>>>(a);
         else k_fused_march<L, S, OUT, false, 3><<<grid, 256, 0, c->stream>>>(a);
@@ -1178,9 +1304,14 @@ int fused_create(mplb_case* c) {
             z->march_keep = !(rp && rp[0] == '1');
             const char* mw = std::getenv("MPLB_MARCH_WAVES");
             z->march_waves = (mw && mw[0] == '3') ? 3 : 4;
+            const char* mt = std::getenv("MPLB_MARCH_TILE");
+            z->march_tile = mt ? std::atoi(mt) : 8;
+            if (z->march_tile != 9 && z->march_tile != 16) z->march_tile = 8;
+            const int ty = z->march_tile == 16 ? 16 : 8;
             z->mtx = (m.nx + MX - 1) / MX;
-            z->mtiles = z->mtx * ((m.ny + MY - 1) / MY);
-            int nchunks = (2048 + z->mtiles - 1) / z->mtiles;
+            z->mtiles = z->mtx * ((m.ny + ty - 1) / ty);
+            const int target = ty == 16 ? 1024 : 2048;
+            int nchunks = (target + z->mtiles - 1) / z->mtiles;
             nchunks = std::max(1, std::min(nchunks, m.nz / 8));
             z->mzc = (m.nz + nchunks - 1) / nchunks;
             nchunks = (m.nz + z->mzc - 1) / z->mzc;

@@ -2,8 +2,10 @@
 // case's own device fields:
 //   mplb_set_radial_density  rho(x) = profile(|x - centre| - R) (linear interpolation,
 //                            clamped ends) written straight into the case's density,
-//                            returning the total mass, so the host can pick R so the
-//                            proposal carries the mass of the cold initial field;
+//                            returning the mass of the distinct lattice sites (a
+//                            periodic axis's last node duplicates its first and is not
+//                            counted), so the host can pick R so the proposal carries
+//                            the mass of the cold initial field;
 //   mplb_density_change      max_x |rho(x) - rho_prev(x)| of the current time level
 //                            against the previous call's snapshot (device reduction).
 #include <hip/hip_runtime.h>
@@ -35,10 +37,11 @@ int wfail(mplb_case* c, int code, const std::string& m) {
 
 struct Radial {
     int64_t N;
-    int nx, ny;
+    int nx, ny, nz;
     double cx, cy, cz, R, d0, dd;
     const double* prof;
     int nprof;
+    int dup;          // bit a set: the last node along axis a duplicates the first (periodic)
 };
 
 // one partial mass per block (FP64, fixed order); the host adds the partials in order
@@ -65,7 +68,8 @@ __global__ void k_radial(Radial a, T* __restrict__ rho, double* __restrict__ par
         }
         const T xv = (T)x;
         rho[id] = xv;
-        v = (double)xv;
+        const bool copy = ((a.dup & 1) && i == a.nx - 1) || ((a.dup & 2) && j == a.ny - 1) || ((a.dup & 4) && k == a.nz - 1);
+        v = copy ? 0.0 : (double)xv;
     }
     s[threadIdx.x] = v;
     __syncthreads();
@@ -107,6 +111,12 @@ int mplb_set_radial_density(mplb_case* c, const double* prof, int nprof, double 
     if (c->opts.device < 0 || !c->rho) return wfail(c, MPLB_E_STATE, "the case has no device fields (device < 0)");
     if (!c->mesh.structured) return wfail(c, MPLB_E_MODEL, "radial fields need a lattice in structured box order");
     WCHK(c, hipSetDevice(c->opts.device));
+    if (c->fused) {
+        // bring the generic fields to the current level first, as mplb_set_field does, so
+        // that later field transfers do not overwrite the density written here
+        const int rc = fused_set_field(c, "rho", nullptr, 0);
+        if (rc < 0) return rc;
+    }
     const int64_t N = c->mesh.N;
     const unsigned nb = (unsigned)((N + kBlock - 1) / kBlock);
     double* dprof = nullptr;
@@ -114,7 +124,20 @@ int mplb_set_radial_density(mplb_case* c, const double* prof, int nprof, double 
     WCHK(c, hipMalloc(&dprof, sizeof(double) * nprof));
     WCHK(c, hipMalloc(&part, sizeof(double) * nb));
     WCHK(c, hipMemcpyAsync(dprof, prof, sizeof(double) * nprof, hipMemcpyHostToDevice, c->stream));
-    Radial a{N, c->mesh.nx, c->mesh.ny, centre[0], centre[1], c->mesh.d == 3 ? centre[2] : 0.0, R, d0, dd, dprof, nprof};
+    // periodic axes of the built boxes (latticeBox2D/3D boundaryType); none for lattices
+    // read from files
+    int dup = 0;
+    switch (c->mesh.btype) {
+    case BOX_PERIODIC_X: dup = 1; break;
+    case BOX_PERIODIC_Y: dup = 2; break;
+    case BOX_PERIODIC_XY: dup = 3; break;
+    case BOX_PERIODIC_Z: dup = 4; break;
+    case BOX_PERIODIC_XYZ: dup = 7; break;
+    case BOX_PERIODIC_XY_SPOT: dup = 3; break;
+    default: dup = 0;
+    }
+    Radial a{N, c->mesh.nx, c->mesh.ny, c->mesh.nz, centre[0], centre[1], c->mesh.d == 3 ? centre[2] : 0.0, R, d0, dd,
+             dprof, nprof, dup};
     if (c->scalar_bytes == 8) k_radial<double><<<nb, kBlock, 0, c->stream>>>(a, (double*)c->rho, part);
     else k_radial<float><<<nb, kBlock, 0, c->stream>>>(a, (float*)c->rho, part);
     WCHK(c, hipGetLastError());

@@ -3,26 +3,36 @@
 A small network (Mlp: BF16 matrix-core products, FP32 accumulation, trained on the GPU
 by the library's own forward/backward/Adam kernels) maps the case parameters to the
 radial profile of the steady density around the interface.  The proposal is written
-onto the lattice with the radius that gives it the same total mass as the case's cold
-initial field, and the solver then runs from it to its own steady criterion: the
-network only chooses the starting point, the solver decides the result, and a warm run
-is checked against the cold run of the same case.
+onto the lattice with the radius that gives its distinct sites the same mass as the
+case's cold initial field, and the solver then runs from it to its own steady criterion:
+the network only chooses the starting point, the solver decides the result, and a warm
+run is checked against the cold run of the same case.
 
-Profile: 64 samples of the density at signed distance d_k = -15.75 + 0.5 k from the
-interface (d < 0 inside), as y_k = (rho - rho_v) / (rho_l - rho_v) with the Maxwell
-densities of the case's temperature; a second block of 64 outputs carries the
-temperature profile as 100 (T / T0 - 1) (zero for isothermal cases).  Beyond
-|d| = 15.75 the end values continue.
+Profile: 64 samples of the density at signed distances DK from the interface (d < 0
+inside; 0.25 apart for |d| < 6, 1.36 apart out to |d| = 16), as
+y = (rho - rho_v) / (rho_l - rho_v) with the Maxwell densities of the case's
+temperature.  The network predicts the correction to the cold start's own profile (tanh
+of width 5 between the Maxwell densities), o = (y - y_cold) / RES_SCALE, so its
+BF16-level relative error lands on a correction of a few percent rather than on the
+whole density jump.  A second block of 64 outputs carries the temperature profile as
+100 (T / T0 - 1) (zero for isothermal cases).  For the lattice the 64 samples are
+interpolated (monotone cubic) onto a uniform 0.05 grid; beyond |d| = 16 the end values
+continue.
 """
 import numpy as np
+from scipy.interpolate import PchipInterpolator
 
 from .initfields import maxwell, vdw_critical
 
 NPROF = 64
-D0, DD = -15.75, 0.5
+DK = np.concatenate([np.linspace(-16.0, -6.5, 8), np.linspace(-5.875, 5.875, 48), np.linspace(6.5, 16.0, 8)])
+FINE_D0, FINE_DD = -16.0, 0.05
+FINE_N = 641
 N_IN = 9
 N_OUT = 2 * NPROF
 WIDTHS = [N_IN, 256, 256, 256, N_OUT]
+RES_SCALE = 0.05
+COLD_WIDTH = 5.0
 
 
 def features(tr, R, L, droplet, dim, a=0.5, b=4.0, G=-1.0, tau=1.0):
@@ -40,51 +50,79 @@ def node_radius(shape, centre):
     return np.sqrt(sum((idx[d] - centre[d]) ** 2 for d in range(len(shape))))
 
 
-def extract_profile(rho, r, rho_v, rho_l, droplet):
-    """Target profile of one steady field: the interface radius where the radially
-    averaged density crosses the mid density, then the density averaged over shells of
-    width 0.5 centred at d_k; returned normalised (y_k)."""
-    rin = rho[r < max(2.0, r.min() + 1.5)].mean()
-    rout = rho[r > r.max() - 2.0].mean()
-    mid = 0.5 * (rin + rout)
-    edges = np.arange(0.0, r.max() + 0.5, 0.5)
+def distinct_sites(shape):
+    """Mask of the distinct sites of a fully periodic box with duplicate end nodes
+    (node-major, x fastest): the last node along each axis repeats the first."""
+    idx = np.indices(shape[::-1]).reshape(len(shape), -1)[::-1]
+    m = np.ones(idx.shape[1], bool)
+    for d, n in enumerate(shape):
+        m &= idx[d] < n - 1
+    return m
+
+
+def _shell_profile(rho, r, width):
+    """Density averaged over shells of the given width, at the shells' mean radius."""
+    edges = np.arange(0.0, r.max() + width, width)
     cnt, _ = np.histogram(r, edges)
     tot, _ = np.histogram(r, edges, weights=rho)
+    rs, _ = np.histogram(r, edges, weights=r)
     ok = cnt > 0
-    rc = 0.5 * (edges[1:] + edges[:-1])[ok]
-    prof = tot[ok] / cnt[ok]
-    s = np.sign(prof - mid)
+    return rs[ok] / cnt[ok], tot[ok] / cnt[ok]
+
+
+def extract_profile(rho, r, rho_v, rho_l, droplet):
+    """Target profile of one steady field: the interface radius where the shell-averaged
+    density crosses the mid density, then the shell-averaged density at distances DK
+    from it, normalised (y).  Raises ValueError when the field has no interface left
+    (the droplet or bubble dissolved)."""
+    rm, pm = _shell_profile(rho, r, 0.05)
+    rin, rout = pm[rm < rm[0] + 1.5].mean(), pm[rm > rm[-1] - 2.0].mean()
+    if abs(rin - rout) < 0.25 * (rho_l - rho_v):
+        raise ValueError("no interface left")
+    mid = 0.5 * (rin + rout)
+    s = np.sign(pm - mid)
     cross = np.nonzero(s[:-1] != s[1:])[0]
     if cross.size == 0:
         raise ValueError("no interface found")
     i = cross[0]
-    R = rc[i] + (mid - prof[i]) * (rc[i + 1] - rc[i]) / (prof[i + 1] - prof[i])
-    d = r - R
-    dk = D0 + DD * np.arange(NPROF)
-    k = np.floor((d - (D0 - 0.5 * DD)) / DD).astype(np.int64)
-    sel = (k >= 0) & (k < NPROF)
-    c2 = np.bincount(k[sel], minlength=NPROF)
-    t2 = np.bincount(k[sel], weights=rho[sel], minlength=NPROF)
-    y = np.empty(NPROF)
-    have = c2 > 0
-    y[have] = t2[have] / c2[have]
-    if not have.all():                 # shells without nodes (small radii): interpolate
-        y[~have] = np.interp(dk[~have], dk[have], y[have])
+    R = rm[i] + (mid - pm[i]) * (rm[i + 1] - rm[i]) / (pm[i + 1] - pm[i])
+    y = np.interp(R + DK, rm, pm)
     return (y - rho_v) / (rho_l - rho_v), R
+
+
+def cold_profile(droplet):
+    """y at DK of the cold start: 0.5 (in + out) + 0.5 (out - in) tanh(2 d / W)."""
+    t = np.tanh(2.0 * DK / COLD_WIDTH)
+    return 0.5 * (1.0 - t) if droplet else 0.5 * (1.0 + t)
+
+
+def target_from_profile(y, droplet):
+    """Network target of a steady profile y: the scaled correction to the cold profile."""
+    return (np.asarray(y, np.float64) - cold_profile(droplet)) / RES_SCALE
+
+
+def profile_from_output(o, droplet):
+    return cold_profile(droplet) + RES_SCALE * np.asarray(o, np.float64)
 
 
 def profile_density(y, rho_v, rho_l):
     return rho_v + (rho_l - rho_v) * np.asarray(y, np.float64)
 
 
-def _interp(prof, d):
-    t = np.clip((d - D0) / DD, 0.0, NPROF - 1.0)
-    q = np.minimum(t.astype(np.int64), NPROF - 2)
+def fine_profile(prof):
+    """The 64 samples at DK interpolated (monotone cubic) onto the uniform fine grid."""
+    d = FINE_D0 + FINE_DD * np.arange(FINE_N)
+    return PchipInterpolator(DK, np.asarray(prof, np.float64))(d)
+
+
+def _interp(fine, d):
+    t = np.clip((d - FINE_D0) / FINE_DD, 0.0, FINE_N - 1.0)
+    q = np.minimum(t.astype(np.int64), FINE_N - 2)
     w = t - q
-    return prof[q] + w * (prof[q + 1] - prof[q])
+    return fine[q] + w * (fine[q + 1] - fine[q])
 
 
-def _secant(mass_of, target, R0, tol=1e-9, iters=30):
+def _secant(mass_of, target, R0, tol=1e-12, iters=40):
     Ra, Rb = R0, R0 * 1.02 + 0.05
     ma, mb = mass_of(Ra) - target, mass_of(Rb) - target
     for _ in range(iters):
@@ -98,17 +136,20 @@ def _secant(mass_of, target, R0, tol=1e-9, iters=30):
     return Rb, mb
 
 
-def raster_host(prof, r, mass, R0):
-    """Density on the nodes at distances r from the centre with the mass `mass`:
-    rho = prof(r - R) with R solved by the secant method (host, numpy)."""
-    R, _ = _secant(lambda R: float(_interp(prof, r - R).sum()), mass, R0)
-    return _interp(prof, r - R), R
+def raster_host(prof, r, mass, R0, sites):
+    """Density on the nodes at distances r from the centre whose distinct sites (mask)
+    carry the mass `mass`: rho = profile(r - R), R from the secant method (numpy)."""
+    fine = fine_profile(prof)
+    R, _ = _secant(lambda R: float(_interp(fine, r[sites] - R).sum()), mass, R0)
+    return _interp(fine, r - R), R
 
 
 def raster_case(case, prof, centre, mass, R0):
-    """Same on a case's device density (mplb_set_radial_density); returns R."""
-    R, _ = _secant(lambda R: case.set_radial_density(prof, D0, DD, centre, R), mass, R0)
-    case.set_radial_density(prof, D0, DD, centre, R)
+    """Same on a case's device density (mplb_set_radial_density, which sums the distinct
+    sites of the case's periodic box); returns R."""
+    fine = fine_profile(prof)
+    R, _ = _secant(lambda R: case.set_radial_density(fine, FINE_D0, FINE_DD, centre, R), mass, R0)
+    case.set_radial_density(fine, FINE_D0, FINE_DD, centre, R)
     return R
 
 

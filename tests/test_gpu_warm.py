@@ -23,19 +23,21 @@ def test_radial_density_matches_host(precision):
     spec, fk, c = d3_case(precision)
     n = spec.nx
     cen = ((n - 1) / 2.0, (n - 1) / 2.0, (n - 1) / 2.0)
-    y = np.tanh(np.linspace(-3, 3, W.NPROF))
-    prof = W.profile_density(0.5 * (1 - y), 0.03, 0.16)
+    y = 0.5 * (1 - np.tanh(W.DK / 2.5))
+    prof = W.profile_density(y, 0.03, 0.16)
+    fine = W.fine_profile(prof)
     r = W.node_radius((n, n, n), cen)
+    sites = W.distinct_sites((n, n, n))            # periodicXYZ: duplicates left out
     for R in (2.5, 3.7, 5.0):
-        mass = c.set_radial_density(prof, W.D0, W.DD, cen, R)
-        host = W._interp(prof, r - R)
+        mass = c.set_radial_density(fine, W.FINE_D0, W.FINE_DD, cen, R)
+        host = W._interp(fine, r - R)
         if precision == M.FP32:
             host = host.astype(np.float32).astype(np.float64)
         got = c.get("rho")
         np.testing.assert_allclose(got, host, rtol=1e-15, atol=0)
-        assert abs(mass - host.sum()) <= 1e-12 * host.sum()
+        assert abs(mass - host[sites].sum()) <= 1e-12 * host.sum()
     # the radius search reproduces a target mass
-    target = float(W._interp(prof, r - 4.2).sum())
+    target = float(W._interp(fine, r - 4.2)[sites].sum())
     R = W.raster_case(c, prof, cen, target, 3.0)
     assert abs(R - 4.2) < 1e-6
 
@@ -58,3 +60,26 @@ def test_density_change_matches_host_difference(fused):
     c.step(3)
     d2 = c.density_change()
     assert d2 == np.max(np.abs(c.get("rho") - after))
+
+
+def test_radial_density_survives_field_transfers_on_the_fused_path():
+    """After steps on the fused path, a radial density written into the case must be
+    what mplb_init starts from, even when U and T are set afterwards."""
+    spec, fk, c = d3_case(M.FP32, fused=True)
+    rho, U, T = sphere_fields(spec, seed=3, **fk)
+    c.set("rho", rho)
+    c.set("U", U)
+    c.set("T", T)
+    c.init()
+    c.step(20)
+    n = spec.nx
+    cen = ((n - 1) / 2.0,) * 3
+    prof = W.profile_density(0.5 * (1 - np.tanh(W.DK / 2.5)), 0.03, 0.16)
+    fine = W.fine_profile(prof)
+    c.set_radial_density(fine, W.FINE_D0, W.FINE_DD, cen, 3.3)
+    c.set("U", np.zeros_like(U))
+    c.set("T", T)
+    expect = W._interp(fine, W.node_radius((n, n, n), cen) - 3.3).astype(np.float32).astype(np.float64)
+    np.testing.assert_array_equal(c.get("rho"), expect)
+    c.init()
+    np.testing.assert_array_equal(c.get("rho"), expect)
