@@ -138,17 +138,25 @@ def _secant(mass_of, target, R0, tol=1e-12, iters=40):
 
 def raster_host(prof, r, mass, R0, sites):
     """Density on the nodes at distances r from the centre whose distinct sites (mask)
-    carry the mass `mass`: rho = profile(r - R), R from the secant method (numpy)."""
+    carry the mass `mass`: rho = profile(r - R), R from the secant method (numpy).
+    When moving the interface cannot give that mass (a flat proposal, or a radius that
+    leaves the box), the remainder is spread uniformly over the sites."""
     fine = fine_profile(prof)
-    R, _ = _secant(lambda R: float(_interp(fine, r[sites] - R).sum()), mass, R0)
+    R, err = _secant(lambda R: float(_interp(fine, r[sites] - R).sum()), mass, R0)
+    if not np.isfinite(R) or not 0.25 * R0 <= R <= 4.0 * R0 or abs(err) > 1e-10 * abs(mass):
+        R = R if np.isfinite(R) and 0.25 * R0 <= R <= 4.0 * R0 else R0
+        fine = fine + (mass - float(_interp(fine, r[sites] - R).sum())) / float(sites.sum())
     return _interp(fine, r - R), R
 
 
-def raster_case(case, prof, centre, mass, R0):
+def raster_case(case, prof, centre, mass, R0, nsites):
     """Same on a case's device density (mplb_set_radial_density, which sums the distinct
-    sites of the case's periodic box); returns R."""
+    sites of the case's periodic box; nsites of them); returns R."""
     fine = fine_profile(prof)
-    R, _ = _secant(lambda R: case.set_radial_density(fine, FINE_D0, FINE_DD, centre, R), mass, R0)
+    R, err = _secant(lambda R: case.set_radial_density(fine, FINE_D0, FINE_DD, centre, R), mass, R0)
+    if not np.isfinite(R) or not 0.25 * R0 <= R <= 4.0 * R0 or abs(err) > 1e-10 * abs(mass):
+        R = R if np.isfinite(R) and 0.25 * R0 <= R <= 4.0 * R0 else R0
+        fine = fine + (mass - case.set_radial_density(fine, FINE_D0, FINE_DD, centre, R)) / float(nsites)
     case.set_radial_density(fine, FINE_D0, FINE_DD, centre, R)
     return R
 

@@ -38,7 +38,7 @@ def test_radial_density_matches_host(precision):
         assert abs(mass - host[sites].sum()) <= 1e-12 * host.sum()
     # the radius search reproduces a target mass
     target = float(W._interp(fine, r - 4.2)[sites].sum())
-    R = W.raster_case(c, prof, cen, target, 3.0)
+    R = W.raster_case(c, prof, cen, target, 3.0, int(sites.sum()))
     assert abs(R - 4.2) < 1e-6
 
 
