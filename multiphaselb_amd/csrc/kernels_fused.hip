@@ -1,9 +1,9 @@
 // Fused structured-box path: FP32 arithmetic, FP32 or FP16-deviation population storage.
 //
 // The state between steps is the post-collision population set of the current step.
-// Isothermal D3Q15 boxes whose links are all the periodic wrap take one single-pass
-// kernel per step (k_fused_march: pull-stream, psi in an LDS ring of planes, force,
-// velocity and collision).  Other boxes take two node-parallel passes (three with heat
+// Isothermal D3Q15 boxes with FP32 storage whose links are all the periodic wrap take
+// one single-pass kernel per step (k_fused_march: pull-stream, psi in an LDS ring of
+// planes, force, velocity and collision).  Other boxes take two node-parallel passes (three with heat
 // transfer):
 //   K1  pull-stream f (and g) from the post-collision buffers, apply the boundary
 //       conditions of the node in registers, and write what the neighbours' stencils
@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -606,11 +607,10 @@ __device__ inline float march_psi(const FArgs& a, int x, int y, int z, float* v)
     return potential<float>(a.eos, rho, T);
 }
 
-// KEEP: the populations pulled for psi at plane z + 1 stay in registers until plane
-// z + 1 collides; otherwise the collision pulls them again (L1/L2 hits), which frees
-// registers for more waves per SIMD.
-template <class L, typename S, bool OUT, bool KEEP, int WAVES>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) k_fused_march(FArgs a) {
+// The populations pulled for psi at plane z + 1 stay in registers (fn) until plane z + 1
+// collides; the register budget is four waves per SIMD.
+template <class L, typename S, bool OUT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_fused_march(FArgs a) {
     __shared__ float ring[4][MSLOT];
     const unsigned wi = (blockIdx.x & 7u) * (unsigned)a.mper + (blockIdx.x >> 3);
     if (wi >= (unsigned)a.mitems) return;
@@ -642,134 +642,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
     auto plane = [&](int zr) {
         const int zw = wrapd(zr, a.nz);
         float* sl = ring[(zr - z0 + 1) & 3];
-        if (edge) sl[(ty + 1) * MHX + tx + 1] = march_psi<L, S>(a, ex, ey, zw, KEEP ? fn : tmp);
+        if (edge) sl[(ty + 1) * MHX + tx + 1] = march_psi<L, S>(a, ex, ey, zw, fn);
         if (ring_site) sl[rhy * MHX + rhx] = march_psi<L, S>(a, rx, ry, zw, tmp);
-    };
-    plane(z0 - 1);
-    plane(z0);
-    if (KEEP) {
-#pragma unroll
-        for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
-    }
-    for (int z = z0; z < z1; z++) {
-        plane(z + 1);
-        __syncthreads();
-        if (owner) {
-            float rho = 0.f;
-            if (KEEP) {
-                rho = rho_sym<L, float>(fc);
-            } else {
-                rho = march_pull<L, S>(a, x, y, z, fc);
-            }
-            float psin[L::Q];
-#pragma unroll
-            for (int q = 0; q < L::Q; q++)
-                psin[q] = ring[(z + L::t.c[q][2] - z0 + 1) & 3][(ty + 1 + L::t.c[q][1]) * MHX + tx + 1 + L::t.c[q][0]];
-            float U[3], Fi[3], Ft[3];
-            collide_node<L>(a, fc, rho, psin, true, true, U, Fi, Ft);
-            const int64_t id = (int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy;
-#pragma unroll
-            for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, fc[q], a.fshift[q]);
-            if (OUT) {
-                const int64_t N = a.N;
-                a.out_rho[id] = rho;
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    a.out_U[d * N + id] = U[d];
-                    a.out_Fi[d * N + id] = Fi[d];
-                    a.out_Ft[d * N + id] = Ft[d];
-                }
-            }
-        }
-        if (KEEP) {
-#pragma unroll
-            for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
-        }
-    }
-}
-
-// Same step with a 32 x TY column and the ring sites spread over groups of G lanes: each
-// lane of a group pulls the pairs k = l, l + G, ... of the site's populations and lane 0
-// adds the pair sums in the order rho_sym uses, so a ring site's psi is bit-identical to
-// the psi its owner computes.  The pull work is balanced over all waves of the block.
-template <class L, typename S, bool OUT, int TY, int G>
-__global__ void __launch_bounds__(32 * TY) __attribute__((amdgpu_waves_per_eu(4))) k_fused_march_b(FArgs a) {
-    constexpr int HX = MX + 2, HY = TY + 2, SLOT = HX * HY, NRING = 2 * HX + 2 * TY;
-    constexpr Pairs<L> P = make_pairs<L>();
-    constexpr int NP = Pairs<L>::n, KPL = (NP + G - 1) / G;
-    static_assert(G * NRING <= 32 * TY && 64 % G == 0, "ring groups must fit the block");
-    __shared__ float ring[4][SLOT];
-    const unsigned wi = (blockIdx.x & 7u) * (unsigned)a.mper + (blockIdx.x >> 3);
-    if (wi >= (unsigned)a.mitems) return;
-    const int chunk = (int)(wi / (unsigned)a.mtiles), tile = (int)(wi - (unsigned)chunk * (unsigned)a.mtiles);
-    const int x0 = (tile % a.mtx) * MX, y0 = (tile / a.mtx) * TY;
-    const int z0 = chunk * a.mzc, z1 = min(z0 + a.mzc, a.nz);
-    const int t = threadIdx.x, tx = t % MX, ty = t / MX;
-    const int x = x0 + tx, y = y0 + ty;
-    const bool owner = x < a.nx && y < a.ny;
-    const bool edge = x <= a.nx && y <= a.ny;
-    const int ex = wrapd(x, a.nx), ey = wrapd(y, a.ny);
-    // ring site of this lane's group
-    const int g = t / G, lane = t % G;
-    int rhx = 0, rhy = 0;
-    if (g < HX) {
-        rhx = g;
-    } else if (g < 2 * HX) {
-        rhx = g - HX;
-        rhy = HY - 1;
-    } else if (g < 2 * HX + TY) {
-        rhy = 1 + (g - 2 * HX);
-    } else {
-        rhx = HX - 1;
-        rhy = 1 + (g - 2 * HX - TY);
-    }
-    const bool ring_site = g < NRING && x0 + rhx - 1 <= a.nx && y0 + rhy - 1 <= a.ny;
-    const int rx = wrapd(x0 + rhx - 1, a.nx), ry = wrapd(y0 + rhy - 1, a.ny);
-    const unsigned nx = (unsigned)a.nx, nxy = (unsigned)a.nxy;
-    const unsigned RX[3] = {(unsigned)wrapd(rx - 1, a.nx), (unsigned)rx, (unsigned)wrapd(rx + 1, a.nx)};
-    const unsigned RY[3] = {(unsigned)wrapd(ry - 1, a.ny) * nx, (unsigned)ry * nx, (unsigned)wrapd(ry + 1, a.ny) * nx};
-    float fc[L::Q], fn[L::Q];
-    auto plane = [&](int zr) {
-        const int zw = wrapd(zr, a.nz);
-        float* sl = ring[(zr - z0 + 1) & 3];
-        if (edge) sl[(ty + 1) * HX + tx + 1] = march_psi<L, S>(a, ex, ey, zw, fn);
-        // ring: pair sums of this lane's pairs
-        const unsigned RZ[3] = {(unsigned)wrapd(zw - 1, a.nz) * nxy, (unsigned)zw * nxy, (unsigned)wrapd(zw + 1, a.nz) * nxy};
-        float part[KPL];
-        float v0 = 0.f;
-#pragma unroll
-        for (int j = 0; j < KPL; j++) part[j] = 0.f;
-        if (ring_site) {
-#pragma unroll
-            for (int j = 0; j < KPL; j++) {
-#pragma unroll
-                for (int ll = 0; ll < G; ll++) {
-                    const int k = j * G + ll;
-                    if (k >= NP || ll != lane) continue;
-                    const int q = P.q[k], r = P.r[k];
-                    const float vq = ld<S>((const char*)a.fin + (size_t)q * a.P * sizeof(S),
-                                           RX[1 - L::t.c[q][0]] + RY[1 - L::t.c[q][1]] + RZ[1 - L::t.c[q][2]],
-                                           a.fshift[q]);
-                    const float vr = ld<S>((const char*)a.fin + (size_t)r * a.P * sizeof(S),
-                                           RX[1 - L::t.c[r][0]] + RY[1 - L::t.c[r][1]] + RZ[1 - L::t.c[r][2]],
-                                           a.fshift[r]);
-                    part[j] = vq + vr;
-                }
-            }
-            if (lane == 0) v0 = ld<S>(a.fin, RX[1] + RY[1] + RZ[1], a.fshift[0]);
-        }
-        // lane 0 adds v0 and the pair sums in pair order (as rho_sym)
-        const int base = (t & 63) - lane;
-        float rho = v0;
-#pragma unroll
-        for (int k = 0; k < NP; k++) {
-            const float sk = __shfl(part[k / G], base + (k % G), 64);
-            rho += sk;
-        }
-        if (ring_site && lane == 0) {
-            const float T = a.Tuniform ? a.Tconst : a.Told[(int64_t)rx + (int64_t)ry * a.nx + (int64_t)zw * a.nxy];
-            sl[rhy * HX + rhx] = potential<float>(a.eos, rho, T);
-        }
     };
     plane(z0 - 1);
     plane(z0);
@@ -783,7 +657,7 @@ __global__ void __launch_bounds__(32 * TY) __attribute__((amdgpu_waves_per_eu(4)
             float psin[L::Q];
 #pragma unroll
             for (int q = 0; q < L::Q; q++)
-                psin[q] = ring[(z + L::t.c[q][2] - z0 + 1) & 3][(ty + 1 + L::t.c[q][1]) * HX + tx + 1 + L::t.c[q][0]];
+                psin[q] = ring[(z + L::t.c[q][2] - z0 + 1) & 3][(ty + 1 + L::t.c[q][1]) * MHX + tx + 1 + L::t.c[q][0]];
             float U[3], Fi[3], Ft[3];
             collide_node<L>(a, fc, rho, psin, true, true, U, Fi, Ft);
             const int64_t id = (int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy;
@@ -853,9 +727,6 @@ struct Fused {
     int face_table = 0;            // faces whose links differ from the periodic wrap
     float fshift[15] = {0}, gshift[15] = {0};
     bool march = false;            // single-pass kernel (isothermal, fully periodic D3Q15)
-    bool march_keep = true;        // keep pulled populations in registers between planes
-    int march_waves = 4;           // register budget: waves per SIMD
-    int march_tile = 8;            // 8: 32 x 8 columns; 9: 32 x 8, balanced ring; 16: 32 x 16
     int mtx = 0, mtiles = 0, mitems = 0, mper = 0, mzc = 0;
 };
 
@@ -936,25 +807,12 @@ FArgs make_args(mplb_case* c) {
     return a;
 }
 
+// The single-pass kernel serves FP32 storage only: with FP16 storage the two-pass
+// kernels are faster (the step is no longer bound by the population bytes).
 template <class L, typename S, bool OUT>
 void launch_march(mplb_case* c, const FArgs& a) {
-    const dim3 grid(8u * (unsigned)a.mper);
-    const Fused* z = F(c);
-    if (z->march_tile == 16) {
-        k_fused_march_b<L, S, OUT, 16, 4><<<grid, 512, 0, c->stream>>>(a);
-        return;
-    }
-    if (z->march_tile == 9) {          // 32 x 8 with balanced ring pulls
-        k_fused_march_b<L, S, OUT, 8, 2><<<grid, 256, 0, c->stream>>>(a);
-        return;
-    }
-    if (z->march_waves == 3) {
-        if (z->march_keep) k_fused_march<L, S, OUT, true, 3><<<grid, 256, 0, c->stream>>>(a);
-        else k_fused_march<L, S, OUT, false, 3><<<grid, 256, 0, c->stream>>>(a);
-    } else {
-        if (z->march_keep) k_fused_march<L, S, OUT, true, 4><<<grid, 256, 0, c->stream>>>(a);
-        else k_fused_march<L, S, OUT, false, 4><<<grid, 256, 0, c->stream>>>(a);
-    }
+    if constexpr (L::D == 3 && std::is_same_v<S, float>)
+        k_fused_march<L, S, OUT><<<dim3(8u * (unsigned)a.mper), 256, 0, c->stream>>>(a);
 }
 
 template <class L, typename S>
@@ -971,7 +829,7 @@ void launch_step(mplb_case* c, FArgs a, bool thermal, bool outputs) {
         k_fused_heat<L><<<grid, block, 0, c->stream>>>(a);
         k_fused_collide<L, S, true><<<grid, block, 0, c->stream>>>(a);
     } else if (F(c)->march) {
-        if constexpr (L::D == 3) launch_march<L, S, false>(c, a);
+        launch_march<L, S, false>(c, a);
     } else {
         k_fused_density<L, S, false><<<grid, block, 0, c->stream>>>(a);
         k_fused_collide<L, S, false><<<grid, block, 0, c->stream>>>(a);
@@ -1021,7 +879,7 @@ void launch_step_timed(mplb_case* c, FArgs a, bool thermal, bool outputs) {
         k_fused_collide<L, S, true><<<grid, block, 0, c->stream>>>(a);
     } else if (F(c)->march) {
         Ev e(c, 6);
-        if constexpr (L::D == 3) launch_march<L, S, false>(c, a);
+        launch_march<L, S, false>(c, a);
     } else {
         {
             Ev e(c, 7);
@@ -1138,7 +996,7 @@ int materialize(mplb_case* c) {
         k_fused_collide<L, S, true><<<grid, block, 0, c->stream>>>(a);
     } else if (z->march) {
         // the step's own kernel with outputs: it rewrites fa with the same values
-        if constexpr (L::D == 3) launch_march<L, S, true>(c, a);
+        launch_march<L, S, true>(c, a);
     } else {
         k_fused_collide<L, S, false><<<grid, block, 0, c->stream>>>(a);
     }
@@ -1290,28 +1148,20 @@ int fused_create(mplb_case* c) {
         }
         z->face_table = mask;
     }
-    // single-pass kernel: isothermal D3Q15 with every link the periodic wrap and no
-    // boundary node (MPLB_FUSED_KERNEL=twopass keeps the two-pass kernels)
+    // single-pass kernel: FP32 storage, isothermal D3Q15 with every link the periodic
+    // wrap and no boundary node (MPLB_FUSED_KERNEL=twopass keeps the two-pass kernels)
     {
         const char* env = std::getenv("MPLB_FUSED_KERNEL");
         const bool twopass = env && std::strcmp(env, "twopass") == 0;
         bool any_bnd = false;
         for (int64_t id = 0; id < N && !any_bnd; id++) any_bnd = m.onbnd[id] != 0;
-        z->march = !twopass && m.lbmodel == 1 && !z->thermal && z->face_table == 0 && !any_bnd && m.nx >= 3 &&
+        z->march = !twopass && !z->half && m.lbmodel == 1 && !z->thermal && z->face_table == 0 && !any_bnd && m.nx >= 3 &&
                    m.ny >= 3 && m.nz >= 3;
         if (z->march) {
-            const char* rp = std::getenv("MPLB_MARCH_REPULL");
-            z->march_keep = !(rp && rp[0] == '1');
-            const char* mw = std::getenv("MPLB_MARCH_WAVES");
-            z->march_waves = (mw && mw[0] == '3') ? 3 : 4;
-            const char* mt = std::getenv("MPLB_MARCH_TILE");
-            z->march_tile = mt ? std::atoi(mt) : 8;
-            if (z->march_tile != 9 && z->march_tile != 16) z->march_tile = 8;
-            const int ty = z->march_tile == 16 ? 16 : 8;
             z->mtx = (m.nx + MX - 1) / MX;
-            z->mtiles = z->mtx * ((m.ny + ty - 1) / ty);
-            const int target = ty == 16 ? 1024 : 2048;
-            int nchunks = (target + z->mtiles - 1) / z->mtiles;
+            z->mtiles = z->mtx * ((m.ny + MY - 1) / MY);
+            // about 2048 work items (z chunks of at least 8 planes) fill the 256 CUs
+            int nchunks = (2048 + z->mtiles - 1) / z->mtiles;
             nchunks = std::max(1, std::min(nchunks, m.nz / 8));
             z->mzc = (m.nz + nchunks - 1) / nchunks;
             nchunks = (m.nz + z->mzc - 1) / z->mzc;
