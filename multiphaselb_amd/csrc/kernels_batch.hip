@@ -258,13 +258,23 @@ __global__ void k_batch_collide(BArgs a) {
     Sf[6] = -Ft[1];
     Sf[7] = T(2) * (U[0] * Ft[0] - U[1] * Ft[1]);
     Sf[8] = U[0] * Ft[1] + U[1] * Ft[0];
+    if constexpr (sizeof(T) == 8) {
+        // the oracle's order: post-collision moments, then populations rebuilt from them
 #pragma unroll
-    for (int q = 0; q < 9; q++) {
-        const T l = T(a.lam[q]);
-        m[q] = m[q] - l * (m[q] - meq[q]) + (T(1) - T(0.5) * l) * Sf[q];
+        for (int q = 0; q < 9; q++) {
+            const T l = T(a.lam[q]);
+            m[q] = m[q] - l * (m[q] - meq[q]) + (T(1) - T(0.5) * l) * Sf[q];
+        }
+        populations<D2Q9, T>(m, v);
+    } else {
+        // the moment change added to the pulled populations (no per-step rebuild)
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const T l = T(a.lam[q]);
+            m[q] = (T(1) - T(0.5) * l) * Sf[q] - l * (m[q] - meq[q]);
+        }
+        populations_add_sym<D2Q9, T>(m, v);
     }
-    if constexpr (sizeof(T) == 8) populations<D2Q9, T>(m, v);
-    else populations_sym<D2Q9, T>(m, v);
 #pragma unroll
     for (int q = 0; q < 9; q++) st<S, T>(a.fout, q * a.P + p.gid, v[q], shift_of<T>(bc, q));
 }

@@ -430,8 +430,9 @@ __device__ inline void collide_node(const FArgs& a, float* v, float rho, const f
         Sf[13] = U[0] * Fu[2] + U[2] * Fu[0];
         Sf[14] = 0.f;
     }
+    // m becomes the moment change of the collision, added to v at the end
 #pragma unroll
-    for (int q = 0; q < L::Q; q++) m[q] = m[q] - a.fl[q] * (m[q] - meq[q]) + (1.f - 0.5f * a.fl[q]) * Sf[q];
+    for (int q = 0; q < L::Q; q++) m[q] = (1.f - 0.5f * a.fl[q]) * Sf[q] - a.fl[q] * (m[q] - meq[q]);
     if (a.surf) {
         // liMRTSurfaceTension.c / xuMRTSurfaceTension.c (zero next to missing links)
         if (full) {
@@ -463,7 +464,7 @@ __device__ inline void collide_node(const FArgs& a, float* v, float rho, const f
             }
         }
     }
-    populations_sym<L, float>(m, v);
+    populations_add_sym<L, float>(m, v);
 }
 
 // K2
@@ -535,18 +536,19 @@ __global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
                 const int i0 = 3 + 2 * pr, i1 = 4 + 2 * pr;
                 r[i0] += a.gl[i1] * (a.gl[i0] / 2.f - 1.f) * d[i1];
             }
+            // n becomes the moment change, added to v below
 #pragma unroll
-            for (int q = 0; q < L::Q; q++) n[q] -= r[q];
+            for (int q = 0; q < L::Q; q++) n[q] = -r[q];
             n[0] += (1.f - 0.5f * a.gl[0]) * Sv;
         } else {
             const float n4 = n[4], n6 = n[6];
 #pragma unroll
-            for (int q = 0; q < L::Q; q++) n[q] = n[q] - a.gl[q] * (n[q] - neq[q]);
+            for (int q = 0; q < L::Q; q++) n[q] = -a.gl[q] * (n[q] - neq[q]);
             n[0] += Sv;
             n[3] += (1.f - a.gl[3] / 2.f) * a.gl[4] * n4;
             n[5] += (1.f - a.gl[5] / 2.f) * a.gl[6] * n6;
         }
-        populations_sym<L, float>(n, v);
+        populations_add_sym<L, float>(n, v);
 #pragma unroll
         for (int q = 0; q < L::Q; q++) st<S>(a.gout, q * a.P + id, v[q], a.gshift[q]);
         a.Unew[id] = U[0];
@@ -608,9 +610,13 @@ __device__ inline float march_psi(const FArgs& a, int x, int y, int z, float* v)
 }
 
 // The populations pulled for psi at plane z + 1 stay in registers (fn) until plane z + 1
-// collides; the register budget is four waves per SIMD.
+// collides; the register budget is MPLB_MARCH_WAVES waves per SIMD (three: about 150
+// VGPRs, no spills; four waves spill and run 13% slower on the 256^3 bubble).
+#ifndef MPLB_MARCH_WAVES
+#define MPLB_MARCH_WAVES 3
+#endif
 template <class L, typename S, bool OUT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_fused_march(FArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPLB_MARCH_WAVES))) k_fused_march(FArgs a) {
     __shared__ float ring[4][MSLOT];
     const unsigned wi = (blockIdx.x & 7u) * (unsigned)a.mper + (blockIdx.x >> 3);
     if (wi >= (unsigned)a.mitems) return;

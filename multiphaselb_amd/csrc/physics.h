@@ -271,6 +271,37 @@ __device__ __host__ inline void populations_sym(const T* m, T* v) {
     }
 }
 
+// v += M^-1 dm in the same even/odd form.  Collisions add the moment change to the
+// pulled populations instead of rebuilding them from the post-collision moments: a
+// rebuild rounds every population afresh each step (about an ulp of f, with a bias that
+// does not average out near equilibrium, so the total mass drifts by ~5e-8 per step in
+// FP32), while f + df rounds only the small change and leaves f exactly where df is
+// below half an ulp.  The rest row of dm is zero for the conserved moments, so the mass
+// changes only by the rounding of the additions.
+template <class L, typename T>
+__device__ __host__ inline void populations_add_sym(const T* dm, T* v) {
+    constexpr Pairs<L> P = make_pairs<L>();
+    T r0 = T(0);
+#pragma unroll
+    for (int j = 0; j < L::Q; j++)
+        if (moment_parity<L>(j) == 1 && L::t.invM[0][j] != 0.0) r0 += T(L::t.invM[0][j]) * dm[j];
+    v[0] += r0;
+#pragma unroll
+    for (int k = 0; k < Pairs<L>::n; k++) {
+        const int q = P.q[k];
+        T E = T(0), O = T(0);
+#pragma unroll
+        for (int j = 0; j < L::Q; j++) {
+            const double c = L::t.invM[q][j];
+            if (c == 0.0) continue;
+            if (moment_parity<L>(j) == 1) E += T(c) * dm[j];
+            else O += T(c) * dm[j];
+        }
+        v[q] += E + O;
+        v[P.r[k]] += E - O;
+    }
+}
+
 static_assert(parity_consistent<D2Q9>() && parity_consistent<D3Q15>(), "moment rows must be even or odd");
 
 }  // namespace mplb

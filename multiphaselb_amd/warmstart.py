@@ -18,6 +18,11 @@ whole density jump.  A second block of 64 outputs carries the temperature profil
 100 (T / T0 - 1) (zero for isothermal cases).  For the lattice the 64 samples are
 interpolated (monotone cubic) onto a uniform 0.05 grid; beyond |d| = 16 the end values
 continue.
+
+A droplet or bubble that dissolves in its closed box ends as a uniform field at the
+box's mean density; its target is that flat profile, so the network also learns which
+cases keep an interface.  A proposal outside the physical density range is not used
+(proposal_ok): the run then starts cold.
 """
 import numpy as np
 from scipy.interpolate import PchipInterpolator
@@ -88,6 +93,31 @@ def extract_profile(rho, r, rho_v, rho_l, droplet):
     R = rm[i] + (mid - pm[i]) * (rm[i + 1] - rm[i]) / (pm[i + 1] - pm[i])
     y = np.interp(R + DK, rm, pm)
     return (y - rho_v) / (rho_l - rho_v), R
+
+
+def steady_profile(rho, r, sites, rho_v, rho_l, droplet, flat_tol=0.05):
+    """Target profile of one steady field, whether it kept its interface or not: the
+    interface profile of extract_profile, or, for a droplet or bubble that dissolved, the
+    flat profile of the field's mean density over the distinct sites (the uniform state a
+    closed box ends in).  Returns (y, kept); raises ValueError for a field that is neither
+    (no interface, yet not uniform within flat_tol of the density jump)."""
+    try:
+        y, _ = extract_profile(rho, r, rho_v, rho_l, droplet)
+        return y, True
+    except ValueError:
+        pass
+    span = float(np.max(rho) - np.min(rho))
+    if not np.isfinite(span) or span > flat_tol * (rho_l - rho_v):
+        raise ValueError("field is neither an interface nor uniform")
+    mean = float(np.mean(rho[sites]))
+    return np.full(NPROF, (mean - rho_v) / (rho_l - rho_v)), False
+
+
+def proposal_ok(prof, rho_v, rho_l):
+    """A proposal the solver may start from: finite, and inside [rho_v / 4, 5 rho_l / 4]
+    (a density at or below zero has no pseudopotential)."""
+    prof = np.asarray(prof, np.float64)
+    return bool(np.isfinite(prof).all() and prof.min() > 0.25 * rho_v and prof.max() < 1.25 * rho_l)
 
 
 def cold_profile(droplet):

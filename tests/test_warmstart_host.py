@@ -1,0 +1,86 @@
+"""Host side of the learned warm start: profile extraction from steady fields (interface
+kept or dissolved), the network target round trip, the proposal range check and the
+mass-matching rasteriser, on synthetic fields whose answers are known by construction."""
+import numpy as np
+import pytest
+
+from multiphaselb_amd import warmstart as W
+
+L = 48
+CEN = ((L - 1) / 2.0, (L - 1) / 2.0)
+RV, RL = 0.02, 0.16
+
+
+def tanh_field(R, droplet, width=4.0):
+    r = W.node_radius((L, L), CEN)
+    t = np.tanh(2.0 * (r - R) / width)
+    y = 0.5 * (1.0 - t) if droplet else 0.5 * (1.0 + t)
+    return RV + (RL - RV) * y, r
+
+
+@pytest.mark.parametrize("droplet", [True, False])
+def test_extract_profile_finds_radius_and_shape(droplet):
+    rho, r = tanh_field(11.3, droplet)
+    y, R = W.extract_profile(rho, r, RV, RL, droplet)
+    # shells of width 0.05 average nodes at slightly different radii: the crossing and the
+    # profile are exact to that resolution
+    assert abs(R - 11.3) < 0.05
+    want = 0.5 * (1.0 - np.tanh(2.0 * W.DK / 4.0)) if droplet else 0.5 * (1.0 + np.tanh(2.0 * W.DK / 4.0))
+    inner = np.abs(W.DK) <= 6.0
+    assert np.max(np.abs(y[inner] - want[inner])) < 0.03
+
+
+def test_steady_profile_of_a_dissolved_case_is_flat_at_the_mean():
+    sites = W.distinct_sites((L, L))
+    rng = np.random.default_rng(3)
+    rho = np.full(L * L, 0.151) + 1e-6 * rng.standard_normal(L * L)
+    r = W.node_radius((L, L), CEN)
+    y, kept = W.steady_profile(rho, r, sites, RV, RL, droplet=False)
+    assert not kept
+    assert np.all(y == y[0])
+    assert abs(RV + (RL - RV) * y[0] - rho[sites].mean()) < 1e-15
+    # and a field that kept its interface gives the interface profile
+    rho2, _ = tanh_field(9.0, True)
+    y2, kept2 = W.steady_profile(rho2, r, sites, RV, RL, droplet=True)
+    assert kept2 and y2[0] > 0.9 and y2[-1] < 0.1
+
+
+def test_steady_profile_rejects_a_field_that_is_neither():
+    sites = W.distinct_sites((L, L))
+    r = W.node_radius((L, L), CEN)
+    # two density levels split along x: no radial interface, not uniform
+    rho = np.where(np.arange(L * L) % L < L // 2, RL, RV).astype(np.float64)
+    rho[(r < 2.0)] = rho[r > r.max() - 2.0].mean()
+    with pytest.raises(ValueError):
+        W.steady_profile(rho, r, sites, RV, RL, droplet=True)
+
+
+@pytest.mark.parametrize("droplet", [True, False])
+def test_target_round_trip(droplet):
+    y = np.linspace(0.0, 1.0, W.NPROF) ** 2
+    o = W.target_from_profile(y, droplet)
+    np.testing.assert_allclose(W.profile_from_output(o, droplet), y, rtol=0, atol=1e-15)
+    # the cold profile itself is the zero correction
+    assert np.all(W.target_from_profile(W.cold_profile(droplet), droplet) == 0.0)
+
+
+def test_proposal_range_check():
+    ok = np.linspace(RV, RL, W.NPROF)
+    assert W.proposal_ok(ok, RV, RL)
+    assert not W.proposal_ok(np.where(np.arange(W.NPROF) == 3, -1e-3, ok), RV, RL)
+    assert not W.proposal_ok(np.where(np.arange(W.NPROF) == 3, np.nan, ok), RV, RL)
+    assert not W.proposal_ok(ok * 1.3, RV, RL)
+
+
+def test_raster_matches_mass_for_interface_and_flat_profiles():
+    sites = W.distinct_sites((L, L))
+    r = W.node_radius((L, L), CEN)
+    prof = W.profile_density(0.5 * (1.0 - np.tanh(2.0 * W.DK / 4.0)), RV, RL)
+    target = float(W._interp(W.fine_profile(prof), r - 10.7)[sites].sum())
+    rho, R = W.raster_host(prof, r, target, 9.0, sites)
+    assert abs(R - 10.7) < 1e-6
+    assert abs(rho[sites].sum() - target) <= 1e-10 * target
+    # a flat proposal cannot move an interface: the mass goes on as a uniform offset
+    flat = np.full(W.NPROF, 0.1)
+    rho2, _ = W.raster_host(flat, r, 0.12 * sites.sum(), 9.0, sites)
+    np.testing.assert_allclose(rho2, 0.12, rtol=1e-12)
