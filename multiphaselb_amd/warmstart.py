@@ -166,6 +166,15 @@ def _secant(mass_of, target, R0, tol=1e-12, iters=40):
     return Rb, mb
 
 
+FLAT_TOL = 0.1
+
+
+def is_flat(prof, rho_v, rho_l):
+    """A proposal without an interface: its density range is below FLAT_TOL of the jump
+    (the network's prediction that the droplet or bubble dissolves)."""
+    return float(np.ptp(np.asarray(prof, np.float64))) < FLAT_TOL * (rho_l - rho_v)
+
+
 def raster_host(prof, r, mass, R0, sites):
     """Density on the nodes at distances r from the centre whose distinct sites (mask)
     carry the mass `mass`: rho = profile(r - R), R from the secant method (numpy).
@@ -179,9 +188,14 @@ def raster_host(prof, r, mass, R0, sites):
     return _interp(fine, r - R), R
 
 
-def raster_case(case, prof, centre, mass, R0, nsites):
+def raster_case(case, prof, centre, mass, R0, nsites, uniform=False):
     """Same on a case's device density (mplb_set_radial_density, which sums the distinct
-    sites of the case's periodic box; nsites of them); returns R."""
+    sites of the case's periodic box; nsites of them); returns R.  uniform: the proposal
+    is a dissolved state (is_flat), written as the uniform density mass / nsites (its
+    residual ripple would only add a transient for the solver to damp); returns None."""
+    if uniform:
+        case.set_radial_density(np.full(FINE_N, mass / nsites), FINE_D0, FINE_DD, centre, R0)
+        return None
     fine = fine_profile(prof)
     R, err = _secant(lambda R: case.set_radial_density(fine, FINE_D0, FINE_DD, centre, R), mass, R0)
     if not np.isfinite(R) or not 0.25 * R0 <= R <= 4.0 * R0 or abs(err) > 1e-10 * abs(mass):

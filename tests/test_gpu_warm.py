@@ -83,3 +83,16 @@ def test_radial_density_survives_field_transfers_on_the_fused_path():
     np.testing.assert_array_equal(c.get("rho"), expect)
     c.init()
     np.testing.assert_array_equal(c.get("rho"), expect)
+
+
+def test_uniform_raster_carries_the_mass():
+    """A dissolved-state proposal is written as the uniform density mass / sites."""
+    spec, fk, c = d3_case(M.FP64)
+    n = spec.nx
+    cen = ((n - 1) / 2.0,) * 3
+    sites = W.distinct_sites((n, n, n))
+    mass = 0.137 * sites.sum() * (1.0 + 1e-3)
+    assert W.raster_case(c, np.full(W.NPROF, 0.1), cen, mass, 3.0, int(sites.sum()), uniform=True) is None
+    got = c.get("rho")
+    assert np.all(got == got[0])
+    assert abs(got[sites].sum() / mass - 1.0) < 1e-14

@@ -84,3 +84,10 @@ def test_raster_matches_mass_for_interface_and_flat_profiles():
     flat = np.full(W.NPROF, 0.1)
     rho2, _ = W.raster_host(flat, r, 0.12 * sites.sum(), 9.0, sites)
     np.testing.assert_allclose(rho2, 0.12, rtol=1e-12)
+
+
+def test_flat_proposals_are_recognised():
+    flat = np.full(W.NPROF, 0.15) + 0.001 * np.sin(np.arange(W.NPROF))
+    assert W.is_flat(flat, RV, RL)
+    step = W.profile_density(0.5 * (1.0 - np.tanh(2.0 * W.DK / 4.0)), RV, RL)
+    assert not W.is_flat(step, RV, RL)
