@@ -135,11 +135,16 @@ int mplb_glibc_rand(unsigned seed, int n, int32_t* out);
  * under dir (IO/writeLatticeMesh.c:20-226, one rank). */
 int mplb_write_lattice(const mplb_case* c, const char* dir);
 /* rho, T, p, U, f, g of the current time level into dir/processor0/<time>/ in the case's
- * dataFormat, asciiRaw or binaryRaw (multiPhasePP.c:291-301 with writeScalarField.c /
- * writeVectorField.c, including their NaN check: MPLB_E_NAN). */
+ * dataFormat (multiPhasePP.c:276-309 with writeScalarField.c / writeVectorField.c,
+ * including their NaN check: MPLB_E_NAN): asciiRaw or binaryRaw, one file per field; or
+ * pvtu (IO/pvtu/writeMeshToPvtu.c, writeScalarToPvtu.c, writeArrayToPvtu.c,
+ * writePvtuExtra.c, writeMainPvd.c): processor0/<time>/fields.vtu, <time>/fields.pvtu and
+ * case.pvd under dir, with the field names of start/initialFields. */
 int mplb_write_fields(mplb_case* c, const char* dir);
-/* dir/processor0/<time>/<name>, comps values per node, in the case's dataFormat; values
- * pass through float as the reference's readers do (readScalarFromAsciiRaw.c). */
+/* Field <name> of time level <time> under dir, comps values per node, in the case's
+ * dataFormat; asciiRaw/binaryRaw values pass through float as the reference's readers do
+ * (readScalarFromAsciiRaw.c); pvtu values are read in double (readScalarFromPvtu.c /
+ * readArrayFromPvtu.c). */
 int mplb_read_field(const mplb_case* c, const char* dir, int time, const char* name, int comps, double* out,
                     size_t n);
 

@@ -145,6 +145,11 @@ def write_case(path, spec: CaseSpec):
             bd += ["    }", ""]
         bd += ["}", "", ""]
     _write(path, "start/boundaries", bd)
+    # the field lists of start/initialFields (read by readVTKInfo.c for pvtu output); the
+    # initial values themselves are set through the library
+    _write(path, "start/initialFields", ["scalarFields", "{", "  rho", "  T", "  p", "}", "", "vectorFields", "{",
+                                         "  U", "}", "", "pdfFields", "{", "  f"] + (["  g"] if spec.g_model is not None else [])
+           + ["}", ""])
     return path
 
 

@@ -104,6 +104,30 @@ bool Dict::integer(const std::string& ename, int df, int* out) const {
     return true;
 }
 
+bool Dict::string_list(const std::string& ename, std::vector<std::string>* out) const {
+    out->clear();
+    std::string cur = content_;
+    size_t s = 0;
+    for (;;) {
+        const size_t e = ename.find('/', s);
+        const std::string tk = ename.substr(s, e == std::string::npos ? std::string::npos : e - s);
+        std::string inner;
+        if (!braced(tk, cur, &inner)) return false;
+        cur = inner;
+        if (e == std::string::npos) break;
+        s = e + 1;
+    }
+    size_t i = 0;
+    while (i < cur.size()) {
+        while (i < cur.size() && (cur[i] == ' ' || cur[i] == '\n')) i++;
+        size_t j = i;
+        while (j < cur.size() && cur[j] != ' ' && cur[j] != '\n') j++;
+        if (j > i) out->push_back(cur.substr(i, j - i));
+        i = j;
+    }
+    return true;
+}
+
 bool Dict::vector(const std::string& ename, std::vector<double>* out) const {
     std::string v;
     out->clear();

@@ -24,6 +24,9 @@ public:
     bool integer(const std::string& ename, int df, int* out) const;
     bool vector(const std::string& ename, std::vector<double>* out) const;
     bool string(const std::string& ename, const std::string& df, std::string* out) const;
+    // lookUpStringList.c: every component of ename is a braced block; the words of the
+    // innermost block are the list
+    bool string_list(const std::string& ename, std::vector<std::string>* out) const;
     const std::string& text() const { return content_; }
     void set_text(const std::string& t) { content_ = t; }
 

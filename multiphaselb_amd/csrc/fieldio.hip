@@ -5,10 +5,16 @@
 //   * macroscopic and population fields of a time level, processor0/<time>/<name>, in
 //     the asciiRaw (one node per line, "%g" values) and binaryRaw (node-major float32)
 //     formats (IO/asciiRaw/*, IO/binaryRaw/*); values read back pass through float as
-//     in readScalarFromAsciiRaw.c / readArrayFromAsciiRaw.c.
+//     in readScalarFromAsciiRaw.c / readArrayFromAsciiRaw.c;
+//   * the pvtu format (IO/pvtu/*): processor0/<time>/fields.vtu with the points, the
+//     VTK cells and every field as an ascii Float32 DataArray, <time>/fields.pvtu
+//     naming it, and case.pvd listing the time folders present.
 #include <hip/hip_runtime.h>
 
+#include <dirent.h>
 #include <sys/stat.h>
+
+#include <algorithm>
 
 #include <cerrno>
 #include <cmath>
@@ -19,6 +25,7 @@
 
 #include "../../include/mplb.h"
 #include "case.h"
+#include "dict.h"
 #include "mesh.h"
 
 using namespace mplb;
@@ -58,6 +65,217 @@ void coords(const Mesh& m, int64_t id, int* p) {
     const int64_t r = id - (int64_t)p[2] * nxy;
     p[1] = (int)(r / m.nx);
     p[0] = (int)(r - (int64_t)p[1] * m.nx);
+}
+
+// VTK cells of the lattice, ctype node ids each: generated for a structured box
+// (latticeBox2D.c:336-356 / latticeBox3D.c:360-392: the quad or hexahedron whose lowest
+// node is (i, j, k)), otherwise those of the vtkCells file the lattice was read with
+// (none when that file is absent).  false: a vtkCells file that does not parse.
+bool vtk_cells(const mplb_case* c, std::vector<long long>* conn, int* ctype) {
+    const Mesh& m = c->mesh;
+    *ctype = m.d == 3 ? 8 : 4;
+    conn->clear();
+    if (!m.points.empty() && !m.structured) {
+        File f(c->dir + "/processor0/lattice/vtkCells", "r");
+        if (!f.f) return true;
+        long long n = 0;
+        int t = 0;
+        if (std::fscanf(f.f, "%lld %d", &n, &t) != 2 || n < 0 || t <= 0 || t > 8) return false;
+        *ctype = t;
+        conn->resize((size_t)(n * t));
+        for (long long& v : *conn)
+            if (std::fscanf(f.f, "%lld", &v) != 1 || v < 0 || v >= m.N) return false;
+        return true;
+    }
+    const long long nx = m.nx, ny = m.ny, nz = m.d == 3 ? m.nz : 1;
+    const long long up = nx * ny;
+    for (long long k = 0; k < (m.d == 3 ? nz - 1 : 1); k++)
+        for (long long j = 0; j < ny - 1; j++)
+            for (long long i = 0; i < nx - 1; i++) {
+                const long long b = i + j * nx + k * up;
+                for (long long v : {b, b + 1, b + nx, b + nx + 1}) conn->push_back(v);
+                if (m.d == 3)
+                    for (long long v : {b + up, b + up + 1, b + up + nx, b + up + nx + 1}) conn->push_back(v);
+            }
+    return true;
+}
+
+// field arrays of a write step (multiPhasePP.c:291-301): rho, T, p, U, f, g
+struct FieldItem {
+    const char* name;
+    int comps;
+    bool vector;
+};
+
+// one field from the device, refused with the reference's message when it holds a NaN
+// (writeScalarField.c / writeVectorField.c)
+int fetch_checked(mplb_case* c, const FieldItem& it, std::vector<double>* h) {
+    h->assign((size_t)c->mesh.N * it.comps, 0.0);
+    const int rc = mplb_get_field(c, it.name, h->data(), h->size());
+    if (rc) return rc;
+    for (double v : *h)
+        if (std::isnan(v)) return ffail(c, MPLB_E_NAN, "Floating point exception. NaN solution");
+    return 0;
+}
+
+// "a b c" with single spaces (writeMeshToPvtu.c prints the scalar and pdf names that way
+// and every vector name followed by a space)
+std::string joined(const std::vector<std::string>& v, bool trailing) {
+    std::string s;
+    for (size_t i = 0; i < v.size(); i++) s += v[i] + (trailing || i + 1 < v.size() ? " " : "");
+    return s;
+}
+
+// IO/writeMeshToVTK.c + writeScalarToPvtu.c + writeArrayToPvtu.c + writePvtuExtra.c +
+// writeMainPvd.c for one rank
+int write_pvtu(mplb_case* c, const std::string& dir, const FieldItem* items, int nitems) {
+    const Mesh& m = c->mesh;
+    const int64_t N = m.N;
+    // readVTKInfo.c: the names listed in start/initialFields
+    std::vector<std::string> sc, vc, pc;
+    Dict init;
+    if (init.open(c->dir + "/start/initialFields")) {
+        init.string_list("scalarFields", &sc);
+        init.string_list("vectorFields", &vc);
+        init.string_list("pdfFields", &pc);
+    }
+    std::vector<long long> conn;
+    int ctype = 0;
+    if (!vtk_cells(c, &conn, &ctype)) return ffail(c, MPLB_E_IO, "Ill defined vtkCells file");
+    const long long ncells = (long long)(conn.size() / (size_t)ctype);
+    std::vector<std::vector<double>> data(nitems);
+    for (int k = 0; k < nitems; k++) {
+        const int rc = fetch_checked(c, items[k], &data[k]);
+        if (rc) return rc;
+    }
+    const std::string t = std::to_string(c->current);
+    const std::string pdir = dir + "/processor0/" + t + "/";
+    if (!mkdirs(pdir) || !mkdirs(dir + "/" + t)) return ffail(c, MPLB_E_IO, "Unable to create time folder");
+    const std::string pd = joined(sc, false), vd = joined(vc, true) + joined(pc, false);
+    {
+        File f(pdir + "fields.vtu", "w");
+        if (!f.f) return ffail(c, MPLB_E_IO, "Unable to write fields.vtu");
+        std::fprintf(f.f, "<?xml version=\"1.0\"?>\n");
+        std::fprintf(f.f, "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\" "
+                          "compressor=\"vtkZLibDataCompressor\">\n");
+        std::fprintf(f.f, "  <UnstructuredGrid>\n");
+        std::fprintf(f.f, "    <Piece NumberOfPoints=\"%lld\" NumberOfCells=\"%lld\">\n", (long long)N, ncells);
+        std::fprintf(f.f, "      <Points>\n");
+        std::fprintf(f.f, "        <DataArray type=\"Float32\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+        for (int64_t i = 0; i < N; i++) {
+            int p[3];
+            if (!m.points.empty()) {
+                p[0] = m.points[3 * i];
+                p[1] = m.points[3 * i + 1];
+                p[2] = m.points[3 * i + 2];
+            } else {
+                coords(m, i, p);
+            }
+            std::fprintf(f.f, "          %.1f %.1f %.1f\n", (double)(float)p[0], (double)(float)p[1], (double)(float)p[2]);
+        }
+        std::fprintf(f.f, "        </DataArray>\n      </Points>\n      <Cells>\n");
+        std::fprintf(f.f, "        <DataArray type=\"Int32\" Name=\"connectivity\" format=\"ascii\">\n");
+        for (long long i = 0; i < ncells; i++) {
+            std::fprintf(f.f, "          ");
+            for (int j = 0; j < ctype; j++) std::fprintf(f.f, "%lld ", conn[(size_t)(i * ctype + j)]);
+            std::fprintf(f.f, "\n");
+        }
+        std::fprintf(f.f, "        </DataArray>\n");
+        std::fprintf(f.f, "        <DataArray type=\"Int32\" Name=\"offsets\" format=\"ascii\">\n");
+        for (long long i = 0; i < ncells; i++) std::fprintf(f.f, "          %lld\n", (i + 1) * ctype);
+        std::fprintf(f.f, "        </DataArray>\n");
+        // the reference labels every cell type 8 (writeMeshToPvtu.c), 2D or 3D
+        std::fprintf(f.f, "        <DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
+        for (long long i = 0; i < ncells; i++) std::fprintf(f.f, "          8\n");
+        std::fprintf(f.f, "        </DataArray>\n      </Cells>\n");
+        std::fprintf(f.f, "      <PointData Scalars=\"%s\" Vectors=\"%s\">\n", pd.c_str(), vd.c_str());
+        for (int k = 0; k < nitems; k++) {
+            const FieldItem& it = items[k];
+            const std::vector<double>& h = data[k];
+            if (!it.vector) {
+                std::fprintf(f.f, "        <DataArray type=\"Float32\" Name=\"%s\" format=\"ascii\">\n", it.name);
+                for (int64_t i = 0; i < N; i++) std::fprintf(f.f, "          %g\n", h[i]);
+            } else {
+                std::fprintf(f.f, "        <DataArray type=\"Float32\" Name=\"%s\"  NumberOfComponents=\"%d\" format=\"ascii\">\n",
+                             it.name, it.comps);
+                for (int64_t i = 0; i < N; i++) {
+                    std::fprintf(f.f, "          ");
+                    for (int j = 0; j < it.comps; j++) std::fprintf(f.f, "%g ", h[(size_t)i * it.comps + j]);
+                    std::fprintf(f.f, "\n");
+                }
+            }
+            std::fprintf(f.f, "        </DataArray>\n");
+        }
+        std::fprintf(f.f, "      </PointData>\n    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n");
+        if (std::ferror(f.f)) return ffail(c, MPLB_E_IO, "Unable to write fields.vtu");
+    }
+    {
+        File f(dir + "/" + t + "/fields.pvtu", "w");
+        if (!f.f) return ffail(c, MPLB_E_IO, "Unable to write fields.pvtu");
+        std::fprintf(f.f, "<?xml version=\"1.0\"?>\n");
+        std::fprintf(f.f, "<VTKFile type=\"PUnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n");
+        std::fprintf(f.f, "  <PUnstructuredGrid GhostLevel=\"0\">\n");
+        std::fprintf(f.f, "    <PPointData Scalars=\"%s\" Vectors=\"%s\">\n", pd.c_str(), vd.c_str());
+        for (const std::string& n : sc) std::fprintf(f.f, "      <PDataArray type=\"Float32\" Name=\"%s\"/>\n", n.c_str());
+        for (const std::string& n : vc)
+            std::fprintf(f.f, "      <PDataArray type=\"Float32\" Name=\"%s\" NumberOfComponents=\"3\"/>\n", n.c_str());
+        for (const std::string& n : pc)
+            std::fprintf(f.f, "      <PDataArray type=\"Float32\" Name=\"%s\" NumberOfComponents=\"%d\"/>\n", n.c_str(), m.Q);
+        std::fprintf(f.f, "    </PPointData>\n    <PPoints>\n");
+        std::fprintf(f.f, "      <PDataArray type=\"Float32\" NumberOfComponents=\"3\"/>\n    </PPoints>\n");
+        std::fprintf(f.f, "    <Piece Source=\"../processor0/%s/fields.vtu\"/>\n", t.c_str());
+        std::fprintf(f.f, "  </PUnstructuredGrid>\n</VTKFile>\n");
+    }
+    // writeMainPvd.c: every entry of the case folder whose name starts with an integer,
+    // in increasing order
+    std::vector<int> times;
+    if (DIR* d = opendir(dir.c_str())) {
+        while (struct dirent* e = readdir(d)) {
+            int v;
+            if (std::sscanf(e->d_name, "%d", &v) == 1) times.push_back(v);
+        }
+        closedir(d);
+    }
+    std::sort(times.begin(), times.end());
+    File f(dir + "/case.pvd", "w");
+    if (!f.f) return ffail(c, MPLB_E_IO, "Unable to write case.pvd");
+    std::fprintf(f.f, "<VTKFile type=\"Collection\" version=\"0.1\" byte_order=\"LittleEndian\">\n  <Collection>\n");
+    for (int v : times) std::fprintf(f.f, "    <DataSet part=\"0\"  timestep=\"%d\" file=\"%d/fields.pvtu\" />\n", v, v);
+    std::fprintf(f.f, "  </Collection>\n</VTKFile>\n");
+    return 0;
+}
+
+// readScalarFromPvtu.c / readArrayFromPvtu.c: after <PointData, the DataArray whose
+// second word is Name="<name>"; its values follow the header's remaining words
+int read_pvtu(mplb_case* c, const std::string& path, const char* name, int comps, double* out, size_t n) {
+    File f(path, "r");
+    if (!f.f) return ffail(c, MPLB_E_IO, "Unable to open " + path);
+    char w[4096];
+    auto word = [&]() { return std::fscanf(f.f, "%4095s", w) == 1; };
+    bool in_points = false;
+    while (word())
+        if (std::strcmp(w, "<PointData") == 0) {
+            in_points = true;
+            break;
+        }
+    if (!in_points) return ffail(c, MPLB_E_IO, "No PointData in " + path);
+    const std::string want = std::string("Name=\"") + name + "\"";
+    for (;;) {
+        bool found = false;
+        while (word())
+            if (std::strcmp(w, "<DataArray") == 0) {
+                found = true;
+                break;
+            }
+        if (!found) return ffail(c, MPLB_E_IO, std::string("Field ") + name + " not found in " + path);
+        if (!word() || !word()) return ffail(c, MPLB_E_IO, "Ill defined " + path);
+        if (want != w) continue;
+        for (int k = 0; k < (comps > 1 ? 2 : 1); k++)
+            if (!word()) return ffail(c, MPLB_E_IO, "Ill defined " + path);
+        for (size_t i = 0; i < n; i++)
+            if (std::fscanf(f.f, "%lf", &out[i]) != 1) return ffail(c, MPLB_E_IO, "Ill defined field in " + path);
+        return 0;
+    }
 }
 
 }  // namespace
@@ -201,29 +419,16 @@ int mplb_write_lattice(const mplb_case* cc, const char* dir) {
         }
     }
     {
-        // VTK cells of a structured box (latticeBox2D.c:336-356 / latticeBox3D.c:360-392)
         File f(base + "vtkCells", "w");
         if (!f.f) return ffail(c, MPLB_E_IO, "Unable to write vtkCells");
-        if (!m.points.empty() && !m.structured) {
-            std::fprintf(f.f, "%d %d\n", 0, m.d == 3 ? 8 : 4);
-        } else if (m.d == 3) {
-            const long long nx = m.nx, ny = m.ny, nz = m.nz;
-            std::fprintf(f.f, "%lld %d\n", (nx - 1) * (ny - 1) * (nz - 1), 8);
-            for (long long k = 0; k < nz - 1; k++)
-                for (long long j = 0; j < ny - 1; j++)
-                    for (long long i = 0; i < nx - 1; i++) {
-                        const long long b = i + j * nx + k * nx * ny, up = nx * ny;
-                        std::fprintf(f.f, "%lld %lld %lld %lld %lld %lld %lld %lld \n", b, b + 1, b + nx, b + nx + 1,
-                                     b + up, b + up + 1, b + up + nx, b + up + nx + 1);
-                    }
-        } else {
-            const long long nx = m.nx, ny = m.ny;
-            std::fprintf(f.f, "%lld %d\n", (nx - 1) * (ny - 1), 4);
-            for (long long j = 0; j < ny - 1; j++)
-                for (long long i = 0; i < nx - 1; i++) {
-                    const long long b = i + j * nx;
-                    std::fprintf(f.f, "%lld %lld %lld %lld \n", b, b + 1, b + nx, b + nx + 1);
-                }
+        std::vector<long long> conn;
+        int ctype = 0;
+        if (!vtk_cells(c, &conn, &ctype)) return ffail(c, MPLB_E_IO, "Ill defined vtkCells file");
+        const long long ncells = (long long)(conn.size() / (size_t)ctype);
+        std::fprintf(f.f, "%lld %d\n", ncells, ctype);
+        for (long long i = 0; i < ncells; i++) {
+            for (int j = 0; j < ctype; j++) std::fprintf(f.f, "%lld ", conn[(size_t)(i * ctype + j)]);
+            std::fprintf(f.f, "\n");
         }
     }
     return 0;
@@ -232,26 +437,19 @@ int mplb_write_lattice(const mplb_case* cc, const char* dir) {
 // writeScalarField.c / writeVectorField.c for rho, T, p, U, f, g (multiPhasePP.c:291-301)
 int mplb_write_fields(mplb_case* c, const char* dir) {
     if (!c || !dir) return MPLB_E_ARG;
+    const FieldItem items[] = {{"rho", 1, false}, {"T", 1, false}, {"p", 1, false},
+                               {"U", 3, true},    {"f", c->mesh.Q, true}, {"g", c->mesh.Q, true}};
+    if (c->data_format == "pvtu") return write_pvtu(c, dir, items, 6);
     const bool binary = c->data_format == "binaryRaw";
     if (!binary && c->data_format != "asciiRaw")
         return ffail(c, MPLB_E_MODEL, "dataFormat " + c->data_format + " is not written by this library");
     const std::string base = std::string(dir) + "/processor0/" + std::to_string(c->current) + "/";
     if (!mkdirs(base)) return ffail(c, MPLB_E_IO, "Unable to create time folder");
     const int64_t N = c->mesh.N;
-    struct Item {
-        const char* name;
-        int comps;
-        bool vector;
-    };
-    const Item items[] = {{"rho", 1, false}, {"T", 1, false}, {"p", 1, false},
-                          {"U", 3, true},    {"f", c->mesh.Q, true}, {"g", c->mesh.Q, true}};
     std::vector<double> h;
-    for (const Item& it : items) {
-        h.assign((size_t)N * it.comps, 0.0);
-        int rc = mplb_get_field(c, it.name, h.data(), h.size());
+    for (const FieldItem& it : items) {
+        const int rc = fetch_checked(c, it, &h);
         if (rc) return rc;
-        for (double v : h)
-            if (std::isnan(v)) return ffail(c, MPLB_E_NAN, "Floating point exception. NaN solution");
         File f(base + it.name, binary ? "wb" : "w");
         if (!f.f) return ffail(c, MPLB_E_IO, std::string("Unable to write ") + it.name);
         if (binary) {
@@ -276,6 +474,8 @@ int mplb_read_field(const mplb_case* cc, const char* dir, int time, const char* 
     mplb_case* c = const_cast<mplb_case*>(cc);
     if (!c || !dir || !name || !out || comps < 1) return MPLB_E_ARG;
     if ((int64_t)n != c->mesh.N * comps) return ffail(c, MPLB_E_ARG, "output size differs from nodes x components");
+    if (c->data_format == "pvtu")
+        return read_pvtu(c, std::string(dir) + "/processor0/" + std::to_string(time) + "/fields.vtu", name, comps, out, n);
     const bool binary = c->data_format == "binaryRaw";
     if (!binary && c->data_format != "asciiRaw")
         return ffail(c, MPLB_E_MODEL, "dataFormat " + c->data_format + " is not read by this library");

@@ -53,3 +53,48 @@ def test_write_fields_nan_check():
     with pytest.raises(M.MplbError) as e:
         c.write_fields(d)
     assert e.value.code == -5 and "NaN" in str(e.value)
+
+
+@pytest.mark.parametrize("name", ["d2q9_bubble_thermal", "d3q15_bubble_nht"])
+def test_pvtu_fields(name):
+    """pvtu output: fields.vtu with the box's points and cells and every field as an
+    ascii DataArray (read back in double, as readScalarFromPvtu.c does with DP), the
+    parallel fields.pvtu naming the fields of start/initialFields, and case.pvd listing
+    the time folders."""
+    spec, fk, ht, frozen, steps = CASES[name]
+    spec = dataclasses.replace(spec, data_format="pvtu")
+    d = make_case_dir(spec)
+    c = M.Case(d, precision=M.FP64, heat_transfer=ht)
+    rho, U, T = sphere_fields(spec, seed=4, **fk)
+    for n, v in (("rho", rho), ("U", U), ("T", T)):
+        c.set(n, v)
+    c.init()
+    c.step(spec.write_interval)
+    t = c.get_info().current_time
+    c.write_fields(d)
+    Q = 9 if spec.lbmodel == 0 else 15
+    for n, comps in (("rho", 1), ("T", 1), ("p", 1), ("U", 3), ("f", Q), ("g", Q)):
+        got = c.read_field(d, t, n, comps)
+        expect = np.vectorize(lambda v: float("%g" % v))(c.get(n))
+        np.testing.assert_array_equal(got, expect.reshape(got.shape))
+    vtu = open(os.path.join(d, "processor0", str(t), "fields.vtu")).read().splitlines()
+    dim3 = spec.lbmodel == 1
+    ncells = (spec.nx - 1) * (spec.ny - 1) * ((spec.nz - 1) if dim3 else 1)
+    assert vtu[3] == '    <Piece NumberOfPoints="%d" NumberOfCells="%d">' % (c.N, ncells)
+    assert vtu[6] == "          0.0 0.0 0.0"
+    assert vtu[7] == "          1.0 0.0 0.0"
+    conn = vtu[6 + c.N + 4].split()
+    if dim3:
+        nx, nxy = spec.nx, spec.nx * spec.ny
+        assert conn == [str(v) for v in (0, 1, nx, nx + 1, nxy, nxy + 1, nxy + nx, nxy + nx + 1)]
+    else:
+        assert conn == ["0", "1", str(spec.nx), str(spec.nx + 1)]
+    assert '      <PointData Scalars="rho T p" Vectors="U f g">' in vtu
+    assert vtu[-1] == "</VTKFile>"
+    pv = open(os.path.join(d, str(t), "fields.pvtu")).read()
+    assert '<PDataArray type="Float32" Name="f" NumberOfComponents="%d"/>' % Q in pv
+    assert '<Piece Source="../processor0/%d/fields.vtu"/>' % t in pv
+    c.step(spec.write_interval)
+    c.write_fields(d)
+    pvd = open(os.path.join(d, "case.pvd")).read()
+    assert pvd.count("<DataSet") == 2 and 'file="%d/fields.pvtu"' % t in pvd
