@@ -733,6 +733,8 @@ struct Fused {
     int face_table = 0;            // faces whose links differ from the periodic wrap
     float fshift[15] = {0}, gshift[15] = {0};
     bool march = false;            // single-pass kernel (isothermal, fully periodic D3Q15)
+    bool t_uniform = false;        // the isothermal temperature field is one value (t_const)
+    float t_const = 0.f;
     int mtx = 0, mtiles = 0, mitems = 0, mper = 0, mzc = 0;
 };
 
@@ -802,7 +804,8 @@ FArgs make_args(mplb_case* c) {
     a.Tnew = z->thermal ? z->T[1 - z->cur] : nullptr;
     a.Uold = z->thermal ? z->U[z->cur] : (const float*)c->U;
     a.Unew = z->thermal ? z->U[1 - z->cur] : nullptr;
-    a.Tuniform = 0;
+    a.Tuniform = (!z->thermal && z->t_uniform) ? 1 : 0;
+    a.Tconst = z->t_const;
     a.mtx = z->mtx;
     a.mtiles = z->mtiles;
     a.mitems = z->mitems;
@@ -1166,9 +1169,23 @@ int fused_create(mplb_case* c) {
         if (z->march) {
             z->mtx = (m.nx + MX - 1) / MX;
             z->mtiles = z->mtx * ((m.ny + MY - 1) / MY);
-            // about 2048 work items (z chunks of at least 8 planes) fill the 256 CUs
-            int nchunks = (2048 + z->mtiles - 1) / z->mtiles;
-            nchunks = std::max(1, std::min(nchunks, m.nz / 8));
+            // z chunks (of at least 8 planes) per column: the work items run in rounds of
+            // `resident` workgroups (CUs x workgroups per CU at the kernel's occupancy), so
+            // pick the count that minimises rounds x (planes per chunk + 2 halo planes)
+            int bpc = 0, cus = 0;
+            FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march<D3Q15, float, false>, 256, 0));
+            FCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->opts.device));
+            const long long resident = std::max(1LL, (long long)bpc * cus);
+            int nchunks = 1;
+            long long best = -1;
+            for (int n = 1; n <= std::max(1, m.nz / 8); n++) {
+                const long long rounds = ((long long)z->mtiles * n + resident - 1) / resident;
+                const long long cost = rounds * ((m.nz + n - 1) / n + 2);
+                if (best < 0 || cost < best) {
+                    best = cost;
+                    nchunks = n;
+                }
+            }
             z->mzc = (m.nz + nchunks - 1) / nchunks;
             nchunks = (m.nz + z->mzc - 1) / z->mzc;
             z->mitems = z->mtiles * nchunks;
@@ -1257,10 +1274,20 @@ int fused_after_write(mplb_case* c) {
     return 0;
 }
 
-int fused_set_field(mplb_case* c, const char*, const double*, size_t) {
+int fused_set_field(mplb_case* c, const char* name, const double* host, size_t n) {
     int rc = fused_sync_to_generic(c);
     if (rc) return rc;
-    F(c)->need_prologue = true;
+    Fused* z = F(c);
+    z->need_prologue = true;
+    if (name && host && std::strcmp(name, "T") == 0 && n == (size_t)c->mesh.N) {
+        // an isothermal step reads T only through psi: a uniform field is passed as one
+        // value (the FP32 value the generic array holds) instead of being read per node
+        const float t0 = (float)host[0];
+        bool uni = true;
+        for (size_t i = 1; i < n && uni; i++) uni = (float)host[i] == t0;
+        z->t_uniform = uni;
+        z->t_const = t0;
+    }
     return 1;
 }
 
