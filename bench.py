@@ -160,6 +160,7 @@ def run_ensemble(args):
         t0 = time.perf_counter()
         b.init()
         block, xcd = b.tune(args.tune_steps)
+        rows = b.launch_info()["march_rows"]
         t1 = time.perf_counter()
         nconv, steps, delta, state = b.run_to_steady(thr[prec], check_every=100, max_steps=args.max_steps)
         t2 = time.perf_counter()
@@ -168,12 +169,13 @@ def run_ensemble(args):
         lu = float(sum(c["L"] ** 2 * int(steps[k]) for k, c in enumerate(ens)))
         res[prec] = dict(steady=int(nconv), diverged=int((state == 2).sum()), threshold=thr[prec],
                          dissolved=int(sum(1 for k in range(len(ens)) if state[k] == 1 and lap[k] is None)),
-                         wall_s=t2 - t0, run_s=t2 - t1, tune_s=t1 - t0, block=block, xcd_order=xcd,
+                         wall_s=t2 - t0, run_s=t2 - t1, tune_s=t1 - t0, block=block, xcd_order=xcd, rows=rows,
                          steps_median=float(np.median(steps)), steps_max=int(steps.max()),
                          mlups=lu / (t2 - t0) / 1e6, lap=lap, state=state)
         b.close()
-        print("# %s: %d/%d steady, %d diverged, %.1f s (tune %.1f s), block %d xcd %d, steps median %d max %d" % (
-            prec, nconv, len(ens), res[prec]["diverged"], t2 - t0, t1 - t0, block, xcd,
+        print("# %s: %d/%d steady, %d diverged, %.1f s (tune %.1f s), %s, xcd %d, steps median %d max %d" % (
+            prec, nconv, len(ens), res[prec]["diverged"], t2 - t0, t1 - t0,
+            "single pass, %d rows per item" % rows if rows else "two passes, block %d" % block, xcd,
             res[prec]["steps_median"], res[prec]["steps_max"]), file=sys.stderr, flush=True)
     ref = res["fp64"]["lap"]
     tc_bins = np.arange(0.6, 0.951, 0.05)
@@ -204,7 +206,8 @@ def run_ensemble(args):
             "dissolved": r["dissolved"], "laplace_cases": len(both),
             "threshold": r["threshold"], "wall_s": round(r["wall_s"], 2), "tune_s": round(r["tune_s"], 2),
             "mlups": round(r["mlups"], 1), "steps_median": r["steps_median"], "steps_max": r["steps_max"],
-            "launch": {"block": r["block"], "xcd_order": r["xcd_order"]},
+            "launch": {"kernel": "k_batch_march" if r["rows"] else "k_batch_psi + k_batch_collide",
+                       "rows_per_item": r["rows"], "block": r["block"], "xcd_order": r["xcd_order"]},
             "laplace_err_vs_fp64": e, "laplace_fit": fits,
             "laplace_fit_max_rel_err_vs_fp64": max([f["rel_err_vs_fp64"] for f in fits], default=None),
         }

@@ -183,11 +183,18 @@ int mplb_batch_run_to_steady(mplb_batch* b, double threshold, int check_every, i
 /* "rho" or "p" (macroPressure.c:20-65, liMRT form) of case i at its current level. */
 int mplb_batch_get_field(mplb_batch* b, int i, const char* name, double* out, size_t n);
 int64_t mplb_batch_case_steps(const mplb_batch* b, int i);
-/* Autotuner: times nodes per workgroup (64..1024) x XCD-ordered work table (0/1) over
- * steps_per_trial steps each on the current cases (which advance meanwhile) and keeps
- * the fastest.  mplb_batch_set_launch sets them by hand. */
+/* Autotuner: times the single-pass kernel at 16/32/64 rows per work item (when every
+ * case is at most 256 nodes wide) and the two-pass kernels at 64..1024 nodes per
+ * workgroup, each with and without the XCD-ordered work table, over steps_per_trial
+ * steps each on the current cases (which advance meanwhile), and keeps the fastest.
+ * chosen_block is the two-pass block size (also used by the measurement passes).
+ * mplb_batch_set_launch selects the two-pass kernels by hand, mplb_batch_set_march the
+ * single-pass kernel (rows per item; 0 = two-pass); both kernels give identical results.
+ * New batches use the single-pass kernel with 32 rows when every case fits. */
 int mplb_batch_tune(mplb_batch* b, int steps_per_trial, int* chosen_block, int* chosen_xcd_order);
 int mplb_batch_set_launch(mplb_batch* b, int block, int xcd_order);
+int mplb_batch_set_march(mplb_batch* b, int rows);
+int mplb_batch_march_rows(const mplb_batch* b);
 int mplb_batch_launch_info(const mplb_batch* b, int* block, int* xcd_order, int* workgroups, int64_t* nodes);
 
 /* ---------------------------------------------------------------------------------

@@ -58,6 +58,7 @@ EXPORTS = [
     "mplb_batch_create", "mplb_batch_close", "mplb_batch_last_error", "mplb_batch_set_case",
     "mplb_batch_init", "mplb_batch_step", "mplb_batch_run_to_steady", "mplb_batch_get_field",
     "mplb_batch_case_steps", "mplb_batch_tune", "mplb_batch_set_launch", "mplb_batch_launch_info",
+    "mplb_batch_set_march", "mplb_batch_march_rows",
     "mplb_density_change", "mplb_set_radial_density",
     "mplb_write_lattice", "mplb_write_fields", "mplb_read_field",
     "mplb_mlp_create", "mplb_mlp_close", "mplb_mlp_last_error", "mplb_mlp_num_params", "mplb_mlp_set_params",
@@ -112,6 +113,8 @@ def library():
         "mplb_batch_case_steps": ([vp, i], C.c_int64),
         "mplb_batch_tune": ([vp, i, C.POINTER(i), C.POINTER(i)], i),
         "mplb_batch_set_launch": ([vp, i, i], i),
+        "mplb_batch_set_march": ([vp, i], i),
+        "mplb_batch_march_rows": ([vp], i),
         "mplb_batch_launch_info": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(C.c_int64)], i),
         "mplb_density_change": ([vp, C.POINTER(d)], i),
         "mplb_write_lattice": ([vp, cp], i),
@@ -387,10 +390,15 @@ class Batch:
     def set_launch(self, block, xcd_order):
         self._check(self.L.mplb_batch_set_launch(self.h, block, 1 if xcd_order else 0))
 
+    def set_march(self, rows):
+        """Single-pass kernel with `rows` rows per work item (0: the two-pass kernels)."""
+        self._check(self.L.mplb_batch_set_march(self.h, rows))
+
     def launch_info(self):
         b, x, w, n = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
         self._check(self.L.mplb_batch_launch_info(self.h, C.byref(b), C.byref(x), C.byref(w), C.byref(n)))
-        return dict(block=b.value, xcd_order=x.value, workgroups=w.value, nodes=n.value)
+        return dict(block=b.value, xcd_order=x.value, workgroups=w.value, nodes=n.value,
+                    march_rows=self.L.mplb_batch_march_rows(self.h))
 
 
 class Mlp:

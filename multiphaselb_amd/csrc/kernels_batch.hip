@@ -7,12 +7,16 @@
 //   * a work table lists (case, first node) per workgroup, so one launch covers cases of
 //     different sizes; its order can give each XCD a contiguous range of the table, so
 //     the rows a workgroup pulls from are mostly in that XCD's L2;
-//   * one step is two launches over the table (psi, then force + velocity + collision with
-//     a pull stream), and runs of steps are captured in a HIP graph and replayed;
+//   * one step is one launch (k_batch_march: workgroups hold whole rows of up to four
+//     cases of one size and march up the rows with psi in an LDS ring, so the
+//     populations are read once per step) or two launches over the node table (psi, then
+//     force + velocity + collision with a pull stream); both give identical results, and
+//     runs of steps are captured in a HIP graph and replayed;
 //   * every `check_every` steps one pass measures each case's largest density change; the
 //     cases that meet the steady criterion leave the work table (their populations stay
 //     in place) and the graph is captured again for the smaller table;
-//   * an autotuner times the nodes per workgroup and the XCD ordering on the live table.
+//   * an autotuner times the launch shapes (rows per march item, or nodes per workgroup
+//     of the two-pass kernels) and the XCD ordering on the live table.
 // Storage: FP64 (same arithmetic as the per-routine FP64 kernels: bit-identical to the
 // reference restatement), FP32, or FP16 deviations f - w_k rho_ref with FP32 arithmetic.
 #include <hip/hip_runtime.h>
@@ -54,6 +58,17 @@ struct BArgs {
     void* p;              // compute type: pressure output
     unsigned* delta;      // per case: max |rho - snapshot| as float bits
 };
+
+// Work item of the single-pass kernel: rows [y0, y1) of up to four cases of the same
+// nx x ny (workgroup lane t serves case slot t / nx, column t % nx); ncase 0 = padding.
+struct MItem {
+    int c[4];
+    int ncase;
+    int nx;
+    int y0, y1;
+};
+
+constexpr int kMarchLanes = 256;
 
 enum Mode : int { FROM_PULL = 0, FROM_INIT = 1 };
 
@@ -166,19 +181,12 @@ __global__ void k_batch_psi(BArgs a) {
 // periodic box is interior), velocity (liMRTVelocity.c:11-50) and the liMRT collision
 // (liMRTCollision.c:33-106 with liMRTForce.c:13-42), written to the other buffer.  The
 // expressions keep the per-routine FP64 kernels' association order.
+// One node: v holds its pulled populations (density rho), psic its psi and psin[q] the
+// psi of x + c_q; the post-collision populations go to node gid of the other buffer.
 template <typename S, typename T, int MODE>
-__global__ void k_batch_collide(BArgs a) {
-    BCase bc;
-    const Pos p = pos(a, bc);
-    if (!p.valid) return;
-    T v[9];
-    const T rho = load_state<S, T, MODE>(a, bc, p, v);
-    const T* psi = (const T*)a.psi;
-    const T psic = psi[p.gid];
+__device__ inline void collide_site(const BArgs& a, const BCase& bc, T* v, const T rho, const T psic, const T* psin,
+                                    int64_t gid) {
     T F[2] = {T(0.0), T(0.0)};
-    T psin[9];
-#pragma unroll
-    for (int q = 1; q < 9; q++) psin[q] = psi[p.off + p.X[1 + D2Q9::t.c[q][0]] + p.Y[1 + D2Q9::t.c[q][1]]];
     constexpr Pairs<D2Q9> PR = make_pairs<D2Q9>();
     if constexpr (sizeof(T) == 8) {
 #pragma unroll
@@ -276,8 +284,83 @@ __global__ void k_batch_collide(BArgs a) {
         populations_add_sym<D2Q9, T>(m, v);
     }
 #pragma unroll
-    for (int q = 0; q < 9; q++) st<S, T>(a.fout, q * a.P + p.gid, v[q], shift_of<T>(bc, q));
+    for (int q = 0; q < 9; q++) st<S, T>(a.fout, q * a.P + gid, v[q], shift_of<T>(bc, q));
 }
+
+template <typename S, typename T, int MODE>
+__global__ void k_batch_collide(BArgs a) {
+    BCase bc;
+    const Pos p = pos(a, bc);
+    if (!p.valid) return;
+    T v[9];
+    const T rho = load_state<S, T, MODE>(a, bc, p, v);
+    const T* psi = (const T*)a.psi;
+    T psin[9];
+#pragma unroll
+    for (int q = 1; q < 9; q++) psin[q] = psi[p.off + p.X[1 + D2Q9::t.c[q][0]] + p.Y[1 + D2Q9::t.c[q][1]]];
+    collide_site<S, T, MODE>(a, bc, v, rho, psi[p.gid], psin, p.gid);
+}
+
+__device__ inline int wrap_dup(int u, int n) { return u < 0 ? u + (n - 1) : (u > n - 1 ? u - (n - 1) : u); }
+
+// The same step in one pass.  Work item it covers rows [y0, y1) of its cases; the
+// workgroup pulls row by row (rows y0 - 1 .. y1, wrapped), turns each row into psi in an
+// LDS ring of four rows and collides row y from the populations it pulled one row
+// earlier (kept in registers) and the psi stencil of rows y - 1 .. y + 1.  A case's row
+// lies whole in the workgroup, so the x stencil wraps inside it (duplicate-node wrap);
+// psi never goes to memory.  Per node and step the populations are read once and written
+// once.  LDS hazards: row y + 1 goes to slot (y + 2 - y0) & 3, which no lane reads while
+// colliding row y - 1 or y, and one barrier per row publishes it.
+template <typename S, typename T>
+__global__ void __launch_bounds__(kMarchLanes) k_batch_march(BArgs a, const MItem* items, int per, int nitems) {
+    __shared__ T ring[4][kMarchLanes];
+    const int wi = (int)((blockIdx.x & 7u) * (unsigned)per + (blockIdx.x >> 3));
+    if (wi >= nitems) return;
+    const MItem it = items[wi];
+    if (it.ncase == 0) return;
+    const int t = threadIdx.x;
+    const int nx = it.nx;
+    const int slot = t / nx, x = t - slot * nx;
+    const bool act = slot < it.ncase;
+    BCase bc{};
+    if (act) bc = a.cases[it.c[slot]];
+    const int ny = act ? bc.ny : 3;
+    const int sb = slot * nx;
+    const unsigned X[3] = {(unsigned)(x == 0 ? nx - 2 : x - 1), (unsigned)x, (unsigned)(x == nx - 1 ? 1 : x + 1)};
+    T fc[9], fn[9];
+    // populations and psi of raw row yr (y0 - 1 <= yr <= y1)
+    auto row = [&](int yr) {
+        if (!act) return;
+        const int yw = wrap_dup(yr, ny);
+        const unsigned Y[3] = {(unsigned)((yw == 0 ? ny - 2 : yw - 1) * nx), (unsigned)(yw * nx),
+                               (unsigned)((yw == ny - 1 ? 1 : yw + 1) * nx)};
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const unsigned l = X[1 - D2Q9::t.c[q][0]] + Y[1 - D2Q9::t.c[q][1]];
+            fn[q] = ld<S, T>(a.fin, q * a.P + bc.off + l, shift_of<T>(bc, q));
+        }
+        ring[(yr - it.y0 + 1) & 3][t] = potential<T>(a.eos, density<T>(fn), T(bc.T));
+    };
+    row(it.y0 - 1);
+    row(it.y0);
+#pragma unroll
+    for (int q = 0; q < 9; q++) fc[q] = fn[q];
+    for (int y = it.y0; y < it.y1; y++) {
+        row(y + 1);
+        __syncthreads();
+        if (act) {
+            T psin[9];
+#pragma unroll
+            for (int q = 1; q < 9; q++)
+                psin[q] = ring[(y + D2Q9::t.c[q][1] - it.y0 + 1) & 3][sb + X[1 + D2Q9::t.c[q][0]]];
+            collide_site<S, T, FROM_PULL>(a, bc, fc, density<T>(fc), ring[(y - it.y0 + 1) & 3][t], psin,
+                                          bc.off + (int64_t)y * nx + x);
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++) fc[q] = fn[q];
+    }
+}
+
 
 // Density of the current time level into the snapshot buffer; with `compare`, the
 // per-case maximum of |rho - previous snapshot| (NaN propagates: its bits order above
@@ -377,6 +460,12 @@ struct mplb_batch {
     unsigned* d_delta = nullptr;
     int bsize = 256;
     int xcd_order = 1;
+    MItem* d_mwork = nullptr;           // single-pass work items (launch order)
+    size_t mwork_cap = 0;
+    int nmwork = 0;                     // launch grid of k_batch_march (8 x mper)
+    int mper = 0;
+    bool march_ok = true;               // every case is at most kMarchLanes wide
+    int mrows = 32;                     // rows per march item; 0: the two-pass kernels
     bool initialized = false;
     bool fields_valid = false;
     int64_t steps = 0;
@@ -436,6 +525,61 @@ int rebuild_work(mplb_batch* b, bool all = false) {
         w = nat;
     }
     b->nwork = (int)w.size();
+    // single-pass items: cases grouped by size, kMarchLanes / nx (at most 4) per item,
+    // mrows rows each; XCD x takes items [x per, (x + 1) per) of the natural order (the
+    // kernel's block remap), or block i takes item i without XCD ordering
+    std::vector<MItem> mw;
+    b->nmwork = 0;
+    b->mper = 0;
+    if (b->mrows > 0) {
+        std::vector<std::pair<int, int>> keys;
+        std::vector<std::vector<int>> members;
+        for (size_t c = 0; c < b->cases.size(); c++) {
+            if (!all && b->cases[c].state != 0) continue;
+            const std::pair<int, int> k(b->cases[c].nx, b->cases[c].ny);
+            size_t g = 0;
+            while (g < keys.size() && keys[g] != k) g++;
+            if (g == keys.size()) {
+                keys.push_back(k);
+                members.emplace_back();
+            }
+            members[g].push_back((int)c);
+        }
+        std::vector<MItem> mnat;
+        for (size_t g = 0; g < keys.size(); g++) {
+            const int nx = keys[g].first, ny = keys[g].second;
+            const int k = std::min(4, kMarchLanes / nx);
+            for (size_t i = 0; i < members[g].size(); i += k) {
+                MItem it{};
+                it.ncase = (int)std::min<size_t>(k, members[g].size() - i);
+                for (int j = 0; j < it.ncase; j++) it.c[j] = members[g][i + j];
+                it.nx = nx;
+                for (int y0 = 0; y0 < ny; y0 += b->mrows) {
+                    it.y0 = y0;
+                    it.y1 = std::min(y0 + b->mrows, ny);
+                    mnat.push_back(it);
+                }
+            }
+        }
+        if (!mnat.empty()) {
+            const size_t per = (mnat.size() + 7) / 8;
+            mw.assign(per * 8, MItem{});
+            for (size_t i = 0; i < mnat.size(); i++) {
+                if (b->xcd_order) mw[i] = mnat[i];
+                else mw[(i % 8) * per + i / 8] = mnat[i];
+            }
+            b->mper = (int)per;
+            b->nmwork = (int)mw.size();
+        }
+    }
+    if (mw.size() > b->mwork_cap) {
+        if (b->d_mwork) BCHK(b, hipFree(b->d_mwork));
+        b->d_mwork = nullptr;
+        BCHK(b, hipMalloc(&b->d_mwork, sizeof(MItem) * mw.size()));
+        b->mwork_cap = mw.size();
+    }
+    if (!mw.empty())
+        BCHK(b, hipMemcpyAsync(b->d_mwork, mw.data(), sizeof(MItem) * mw.size(), hipMemcpyHostToDevice, b->stream));
     if (w.size() > b->work_cap) {
         if (b->d_work) BCHK(b, hipFree(b->d_work));
         b->d_work = nullptr;
@@ -458,8 +602,14 @@ void launch_steps(mplb_batch* b, int nsteps) {
     for (int s = 0; s < nsteps; s++) {
         a.fin = b->fa;
         a.fout = b->fb;
-        k_batch_psi<S, T, FROM_PULL><<<grid, block, 0, b->stream>>>(a);
-        k_batch_collide<S, T, FROM_PULL><<<grid, block, 0, b->stream>>>(a);
+        if (b->mrows > 0) {
+            if (b->nmwork > 0)
+                k_batch_march<S, T><<<(unsigned)b->nmwork, kMarchLanes, 0, b->stream>>>(a, b->d_mwork, b->mper,
+                                                                                      b->nmwork);
+        } else if (b->nwork > 0) {
+            k_batch_psi<S, T, FROM_PULL><<<grid, block, 0, b->stream>>>(a);
+            k_batch_collide<S, T, FROM_PULL><<<grid, block, 0, b->stream>>>(a);
+        }
         std::swap(b->fa, b->fb);
     }
 }
@@ -592,6 +742,7 @@ int mplb_batch_create(mplb_case* const* cases, int n, int precision, int device,
                    std::memcmp(b->lam, mp.Lambda, sizeof(b->lam)) != 0 || b->sigma != mp.sigma) {
             return bfail(b, MPLB_E_MODEL, who + "batched cases share the EOS block and the f model constants");
         }
+        if (m.nx > kMarchLanes) b->march_ok = false;
         BCase bc{};
         bc.nx = m.nx;
         bc.ny = m.ny;
@@ -603,6 +754,7 @@ int mplb_batch_create(mplb_case* const* cases, int n, int precision, int device,
     }
     b->case_steps.assign(n, 0);
     b->P = off;
+    if (!b->march_ok) b->mrows = 0;
     BCHK(b, hipSetDevice(device));
     BCHK(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     BCHK(b, hipMalloc(&b->fa, b->sbytes * 9 * b->P));
@@ -677,8 +829,16 @@ int mplb_batch_tune(mplb_batch* b, int steps_per_trial, int* chosen_block, int* 
     BCHK(b, hipEventCreate(&e0));
     BCHK(b, hipEventCreate(&e1));
     const int trial = std::max(2, steps_per_trial & ~1);    // even: the buffers end where they started
+    // candidates: rows per single-pass item (when every case fits a workgroup row), then
+    // nodes per workgroup of the two-pass kernels
+    const int rows_cand[] = {16, 32, 64};
+    int best_r = b->mrows;
+    const int bs0 = b->bsize;
     for (int xo = 0; xo < 2; xo++) {
-        for (int cb : cand) {
+        for (int ci = b->march_ok ? 0 : 3; ci < 8; ci++) {
+            const int cr = ci < 3 ? rows_cand[ci] : 0;
+            const int cb = ci < 3 ? bs0 : cand[ci - 3];
+            b->mrows = cr;
             b->bsize = cb;
             b->xcd_order = xo;
             int rc = rebuild_work(b);
@@ -698,6 +858,7 @@ int mplb_batch_tune(mplb_batch* b, int steps_per_trial, int* chosen_block, int* 
                 best = ms;
                 best_b = cb;
                 best_x = xo;
+                best_r = cr;
             }
         }
     }
@@ -705,6 +866,7 @@ int mplb_batch_tune(mplb_batch* b, int steps_per_trial, int* chosen_block, int* 
     BCHK(b, hipEventDestroy(e1));
     b->bsize = best_b;
     b->xcd_order = best_x;
+    b->mrows = best_r;
     b->fields_valid = false;
     if (chosen_block) *chosen_block = best_b;
     if (chosen_xcd_order) *chosen_xcd_order = best_x;
@@ -717,10 +879,24 @@ int mplb_batch_set_launch(mplb_batch* b, int block, int xcd_order) {
         return bfail(b, MPLB_E_ARG, "block must be a multiple of 64 in [64, 1024]");
     b->bsize = block;
     b->xcd_order = xcd_order ? 1 : 0;
+    b->mrows = 0;
     if (!b->initialized) return 0;
     BCHK(b, hipSetDevice(b->dev));
     return rebuild_work(b);
 }
+
+int mplb_batch_set_march(mplb_batch* b, int rows) {
+    if (!b) return MPLB_E_ARG;
+    if (rows < 0) return bfail(b, MPLB_E_ARG, "rows must be >= 0");
+    if (rows > 0 && !b->march_ok)
+        return bfail(b, MPLB_E_MODEL, "the single-pass kernel needs every case at most 256 nodes wide");
+    b->mrows = rows;
+    if (!b->initialized) return 0;
+    BCHK(b, hipSetDevice(b->dev));
+    return rebuild_work(b);
+}
+
+int mplb_batch_march_rows(const mplb_batch* b) { return b ? b->mrows : MPLB_E_ARG; }
 
 int mplb_batch_run_to_steady(mplb_batch* b, double threshold, int check_every, int64_t max_steps,
                              int64_t* case_steps, double* case_delta, int* case_state) {
@@ -831,7 +1007,7 @@ int mplb_batch_close(mplb_batch* b) {
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->graph) hipGraphDestroy(b->graph);
-    void* ptrs[] = {b->fa, b->fb, b->psi, b->rho, b->p, b->d_cases, b->d_work, b->d_delta};
+    void* ptrs[] = {b->fa, b->fb, b->psi, b->rho, b->p, b->d_cases, b->d_work, b->d_mwork, b->d_delta};
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (b->stream) hipStreamDestroy(b->stream);

@@ -101,12 +101,16 @@ def test_graph_replay_equals_plain_steps():
     """run_to_steady (graph-captured runs, threshold 0 so no case stops) gives exactly the
     plain launches' state, for each launch shape the tuner can pick."""
     plain, _, _ = make_batch(SMALL, M.FP32)
+    plain.set_launch(256, 1)
     plain.init()
     plain.step(120)
-    for block, xcd in ((64, 0), (256, 1), (1024, 1)):
+    for block, xcd, rows in ((64, 0, 0), (256, 1, 0), (1024, 1, 0), (256, 0, 7), (256, 1, 32)):
         g, _, _ = make_batch(SMALL, M.FP32)
         g.set_launch(block, xcd)
+        if rows:
+            g.set_march(rows)
         g.init()
+        assert g.launch_info()["march_rows"] == rows
         nconv, steps, delta, state = g.run_to_steady(0.0, check_every=40, max_steps=120)
         assert nconv == 0 and (state == 0).all() and (steps == 121).all()
         for k in range(len(SMALL)):
@@ -146,10 +150,11 @@ def test_autotuner_keeps_results_exact():
     assert block in (64, 128, 256, 512, 1024) and xcd in (0, 1)
     info = b.launch_info()
     assert info["block"] == block and info["nodes"] == sum(nx * ny for nx, ny, *_ in SMALL[:2])
+    assert info["march_rows"] in (0, 16, 32, 64)
     b.step(3)
     for k, (nx, ny, *_rest) in enumerate(SMALL[:2]):
         t = b.case_steps(k)
-        assert t == 1 + 10 * (2 + 4) + 3
+        assert t == 1 + 16 * (2 + 4) + 3       # 2 x (3 single-pass + 5 two-pass) trials
         r_ref, _ = oracle_run(nx, ny, rhos[k], temps[k], t)
         np.testing.assert_array_equal(b.get(k, "rho"), r_ref)
 
@@ -169,3 +174,25 @@ def test_fp32_batch_mass_does_not_drift():
         m = float(np.asarray(b.get(k, "rho"), np.float64)[sites].sum())
         print("  case %d mass rel change %.2e" % (k, m / m0 - 1.0), flush=True)
         assert abs(m / m0 - 1.0) < 1e-5
+
+
+@pytest.mark.parametrize("precision", [M.FP64, M.FP32, M.FP16S])
+def test_single_pass_equals_two_pass(precision):
+    """The single-pass kernel (rows of up to four equal-size cases per workgroup, psi in an
+    LDS ring) does each node's arithmetic exactly as the two-pass kernels: identical
+    states, for uneven cases, several cases per item and row counts that do not divide
+    the case height."""
+    cases = SMALL + [(16, 16, 0.8, 4.5, False), (16, 16, 0.75, 3.0, True), (16, 16, 0.9, 5.0, True),
+                     (16, 16, 0.65, 4.0, False), (16, 16, 0.7, 3.5, True)]
+    two, _, _ = make_batch(cases, precision)
+    two.set_launch(128, 1)
+    two.init()
+    two.step(60)
+    for rows in (5, 64):
+        one, _, _ = make_batch(cases, precision)
+        one.set_march(rows)
+        one.init()
+        one.step(60)
+        for k in range(len(cases)):
+            np.testing.assert_array_equal(one.get(k, "rho"), two.get(k, "rho"))
+            np.testing.assert_array_equal(one.get(k, "p"), two.get(k, "p"))
