@@ -148,3 +148,41 @@ def test_nan_at_write_step_is_reported():
         c.step(steps)
     assert e.value.code == -5
     assert o.run(steps) == -1
+
+
+def _maxwell(tr):
+    from multiphaselb_amd.initfields import maxwell
+    return maxwell(tr * TC, a=0.5, b=4.0)
+
+
+# The benchmark configurations' own shapes, 1000 steps in FP64: the D2Q9 liMRT droplet
+# at full size (128^2, R = 30, T = 0.85 Tc), the D3Q15 bubble (xuMRT scheme, T = 0.8 Tc) and
+# the D3Q15 thermal column with gravity, bounceBack floor and ceiling, fixedRandomT floor
+# and fixedT ceiling (xuMRT + myMRT) at reduced sizes, all from Maxwell densities.
+LONG = {
+    "c0_droplet_128": (d2(nx=128, ny=128, btype=3, write_interval=500),
+                       dict(T=0.85 * TC, r_in=_maxwell(0.85)[1], r_out=_maxwell(0.85)[0], radius=30.0), False),
+    "c1_bubble_40": (d3(nx=40, ny=40, nz=40, btype=5, write_interval=500),
+                     dict(T=0.8 * TC, r_in=_maxwell(0.8)[0], r_out=_maxwell(0.8)[1], radius=6.25), False),
+    "c2_column_16x16x48": (d3(nx=16, ny=16, nz=48, btype=3, g=(0.0, 0.0, -1e-5), write_interval=500,
+                              f_bc={"Z0": BC("bounceBack"), "Z1": BC("bounceBack")},
+                              g_bc={"Z0": BC("fixedRandomT", (0.95 * TC,), pert=0.5, duration=10 ** 6),
+                                    "Z1": BC("fixedT", (0.8 * TC,))}),
+                           dict(T=0.8 * TC, r_in=_maxwell(0.8)[1], r_out=_maxwell(0.8)[0], radius=0, column=16.0),
+                           True),
+}
+
+
+@pytest.mark.parametrize("name", list(LONG))
+def test_config_shapes_1000_steps_match_oracle(name):
+    spec, fk, ht = LONG[name]
+    print("case", name, flush=True)
+    fields = sphere_fields(spec, seed=11, **fk)
+    c, o = pair(spec, fields, precision=M.FP64, ht=ht)
+    assert c.step(1000) == 1000 and o.run(1000) == 1000
+    assert o.error()[0] == 0, o.error()
+    for f in ("rho", "U", "T", "f", "g", "p"):
+        a, b = c.get(f), o.get(f)
+        r = max_rel(a, b)
+        print("  %-3s max_rel %.3e exact %s" % (f, r, bool(np.array_equal(a, b))), flush=True)
+        assert np.isfinite(a).all() and r <= TOL, (f, r)
