@@ -492,6 +492,127 @@ def run_warmstart(args):
     return 0
 
 
+def run_accuracy(args):
+    """Reduced precision against FP64 on the configurations' own sizes.  The FP64
+    per-routine path is bit-identical to the oracle (1000-step parity tests), so it
+    stands in for the CPU oracle at sizes the oracle cannot finish in minutes.
+    C1: 256^3 static bubble, 2000 steps: coexistence densities (inside r < R/2, outside
+    r > R + 20), Laplace pressure jump between the same regions, spurious currents (max
+    |U|).  C2: 128 x 128 x 384 thermal column (gravity -1e-5, pool in the lower third,
+    bounceBack floor and ceiling, fixedRandomT floor at 0.95 Tc +-0.5%, fixedT ceiling at
+    0.8 Tc): liquid and vapour densities (z in [20, 100] and [160, 360]), the
+    plane-averaged temperature profile, max |U|.  The thermally coupled column goes
+    unstable in FP64 (the reference arithmetic) after about a thousand steps, heated or
+    not, so the comparison covers the first --acc-c2-steps steps, and the step by which
+    the FP64 run holds NaN is reported."""
+    import multiphaselb_amd as M
+    from multiphaselb_amd.casefiles import BC, CaseSpec
+    from multiphaselb_amd.initfields import maxwell, noise, vdw_critical
+
+    log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
+    tc = vdw_critical(0.5, 4.0)[1]
+    modes = [("fp64", M.FP64, False), ("fp32", M.FP32, True), ("fp16s", M.FP16S, True)]
+
+    def run(d, N, rho, T, prec, fused, ht, steps):
+        t0 = time.perf_counter()
+        c = M.Case(d, precision=prec, heat_transfer=ht, fused=fused, extensions=M.EXT_LIMRT_D3Q15)
+        c.set("rho", rho)
+        c.set("U", np.zeros((N, 3)))
+        c.set("T", T)
+        c.init()
+        done = c.step(steps)
+        out = {k: np.asarray(c.get(k)) for k in ("rho", "U", "p", "T")}
+        c.close()
+        return done, out, time.perf_counter() - t0
+
+    def rel(a, b):
+        return float(abs(a - b) / abs(b)) if b != 0 else float("nan")
+
+    res = {}
+    # C1
+    n, steps1 = args.acc_size, 2000
+    d, spec, rho, T, dens = make_case(n, False, steps1)
+    spec.write_interval = steps1
+    M.write_case(d, spec)
+    N = n ** 3
+    c0 = (n - 1) / 2.0
+    R = 40.0 * n / 256.0
+    idx = np.indices((n, n, n)).reshape(3, -1)[::-1]
+    r = np.sqrt(((idx - c0) ** 2).sum(0))
+    inner, outer = r < 0.5 * R, r > R + 20.0 * n / 256.0
+    obs = {}
+    for name, prec, fused in modes:
+        done, f, wall = run(d, N, rho, np.full(N, T), prec, fused, False, steps1)
+        um = np.sqrt((f["U"] ** 2).sum(1))
+        obs[name] = dict(rho_in=float(f["rho"][inner].mean()), rho_out=float(f["rho"][outer].mean()),
+                         dp=float(f["p"][inner].mean() - f["p"][outer].mean()), umax=float(um.max()),
+                         finite=bool(np.isfinite(f["rho"]).all()), steps=int(done), wall_s=round(wall, 1))
+        log("C1 %s: %s" % (name, obs[name]))
+    ref = obs["fp64"]
+    res["c1_bubble"] = {"size": [n, n, n], "steps": steps1, "fp64": ref,
+                        **{k: {**obs[k], "rel_err_vs_fp64": {q: rel(obs[k][q], ref[q]) for q in
+                                                             ("rho_in", "rho_out", "dp", "umax")}}
+                           for k in ("fp32", "fp16s")}}
+    # C2
+    nx, nz, steps2 = args.acc_size // 2, (3 * args.acc_size) // 2, args.acc_c2_steps
+    rv, rl = maxwell(0.8 * tc, a=0.5, b=4.0)
+    spec2 = CaseSpec(lbmodel=1, nx=nx, ny=nx, nz=nz, btype=3, f_model=0, f_Lambda=LAM15, f_sigma=0.125,
+                     g_model=1, g_Lambda=GLAM15, g_alpha1=0.25, g_alpha2=0.25, g=(0.0, 0.0, -1e-5),
+                     end=10 ** 8, write_interval=steps2,
+                     f_bc={"Z0": BC("bounceBack"), "Z1": BC("bounceBack")},
+                     g_bc={"Z0": BC("fixedRandomT", (0.95 * tc,), pert=0.5, duration=10 ** 8),
+                           "Z1": BC("fixedT", (0.8 * tc,))})
+    d2 = tempfile.mkdtemp(prefix="mplb_acc_")
+    M.write_case(d2, spec2)
+    N2 = nx * nx * nz
+    k = np.arange(nz)
+    prof = 0.5 * (rl + rv) + 0.5 * (rv - rl) * np.tanh(2.0 * (k - nz / 3.0) / 5.0)
+    rho2 = (np.broadcast_to(prof[:, None, None], (nz, nx, nx)) + noise((nz, nx, nx), 1e-4, 20240602,
+                                                                       (True, True, False))).ravel()
+    zi = np.repeat(k, nx * nx)
+    liq, vap = (zi >= 20 * nz // 384) & (zi < 100 * nz // 384), (zi >= 160 * nz // 384) & (zi < 360 * nz // 384)
+    obs2, tprof = {}, {}
+    for name, prec, fused in modes:
+        done, f, wall = run(d2, N2, rho2, np.full(N2, 0.8 * tc), prec, fused, True, steps2)
+        um = np.sqrt((f["U"] ** 2).sum(1))
+        tprof[name] = f["T"].reshape(nz, -1).mean(1)
+        obs2[name] = dict(rho_liquid=float(f["rho"][liq].mean()), rho_vapour=float(f["rho"][vap].mean()),
+                          umax=float(um.max()), finite=bool(np.isfinite(f["rho"]).all()), steps=int(done),
+                          wall_s=round(wall, 1))
+        log("C2 %s: %s" % (name, obs2[name]))
+    ref2 = obs2["fp64"]
+    # the FP64 column continued in runs of 250 steps until it holds NaN (up to 5000)
+    c = M.Case(d2, precision=M.FP64, heat_transfer=True, fused=False, extensions=M.EXT_LIMRT_D3Q15)
+    c.set("rho", rho2)
+    c.set("U", np.zeros((N2, 3)))
+    c.set("T", np.full(N2, 0.8 * tc))
+    c.init()
+    nan_by = None
+    for k2 in range(20):
+        try:
+            c.step(250)
+            ok = bool(np.isfinite(np.asarray(c.get("rho"))).all())
+        except M.MplbError:
+            ok = False
+        if not ok:
+            nan_by = 250 * (k2 + 1)
+            break
+    c.close()
+    log("C2 fp64 holds NaN by step %s" % nan_by)
+    res["c2_column"] = {"size": [nx, nx, nz], "steps": steps2, "fp64_nan_by_step": nan_by, "fp64": ref2,
+                        **{k: {**obs2[k], "rel_err_vs_fp64": {
+                            **{q: rel(obs2[k][q], ref2[q]) for q in ("rho_liquid", "rho_vapour", "umax")},
+                            "T_profile_max": float(np.max(np.abs(tprof[k] - tprof["fp64"]) / tprof["fp64"]))}}
+                           for k in ("fp32", "fp16s")}}
+    out = {"metric": "reduced precision vs FP64 on the configuration sizes", "value": None, "unit": "relative error",
+           "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32, f16-storage/f32 vs f64", "data": "synthetic",
+           "config": {"workload": "C1 bubble %d^3 2000 steps; C2 thermal column %dx%dx%d %d steps" % (
+               n, nx, nx, nz, steps2)}, **res}
+    print(json.dumps(out))
+    return 0
+
+
 def copy_probe(mib, reps=5):
     """A plain device-to-device copy of known size; its FETCH_SIZE / WRITE_SIZE counter
     readings give the scale factor applied to the lattice kernels' counters."""
@@ -517,7 +638,9 @@ def main():
     ap.add_argument("--thermal", action="store_true")
     ap.add_argument("--generic", action="store_true", help="per-routine kernels instead of the fused kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="bubble3d", choices=["bubble3d", "ensemble", "warmstart"])
+    ap.add_argument("--workload", default="bubble3d", choices=["bubble3d", "ensemble", "warmstart", "accuracy"])
+    ap.add_argument("--acc-size", type=int, default=256, help="accuracy: bubble edge (the column is n/2 x n/2 x 3n/2)")
+    ap.add_argument("--acc-c2-steps", type=int, default=750, help="accuracy: column steps compared")
     ap.add_argument("--ws-2d", type=int, default=4096, help="warmstart: 2D training cases")
     ap.add_argument("--ws-3d", type=int, default=256, help="warmstart: 3D training cases")
     ap.add_argument("--ws-eval", type=int, default=256, help="warmstart: held-out 3D bubbles")
@@ -549,6 +672,8 @@ def main():
         return run_ensemble(args)
     if args.workload == "warmstart":
         return run_warmstart(args)
+    if args.workload == "accuracy":
+        return run_accuracy(args)
 
     r = run_gpu(args)
     N, steps = r["N"], args.steps
