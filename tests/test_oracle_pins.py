@@ -226,3 +226,35 @@ def test_heat_source_divergence_term_closed_form():
     expect = b * T * (1.0 - dpdT / rho)
     ss = o.get("ss").reshape(7, 9)
     assert np.allclose(ss[1:-1, 1:-1], expect, rtol=1e-9)
+
+
+@pytest.mark.parametrize("lbmodel", [0, 1])
+def test_heat_source_conduction_term_closed_form(lbmodel):
+    """U = 0, T = T0 + a x and rho = r0 + b x: on interior nodes S = -lambda a g / Cv, with
+    g = (1/rho(x+1) - 1/rho(x-1)) / 2 and the myMRT conductivity lambda = rho Cv (1/L3 - 1/2)
+    (4 + 3 a1 + 2 a2) / 6 (D2Q9) or (6 + 11 a1 + a2) / 9 (D3Q15); the divergence term is
+    zero.  Pins the conductivity coefficients and the orientation of both gradients."""
+    a1, a2, cv = 0.3, 0.7, 1.7
+    if lbmodel == 0:
+        spec = CaseSpec(lbmodel=0, nx=9, ny=7, btype=1, f_model=0, f_Lambda=LAM9, g_model=1, g_Lambda=GLAM9,
+                        g_alpha1=a1, g_alpha2=a2, Cv=cv)
+        shape, lam3, fac = (7, 9), GLAM9[3], (4.0 + 3.0 * a1 + 2.0 * a2) / 6.0
+    else:
+        spec = CaseSpec(lbmodel=1, nx=9, ny=6, nz=5, btype=0, f_model=2, f_Lambda=LAM15, g_model=1,
+                        g_Lambda=GLAM15, g_alpha1=a1, g_alpha2=a2, Cv=cv)
+        shape, lam3, fac = (5, 6, 9), GLAM15[3], (6.0 + 11.0 * a1 + a2) / 9.0
+    o = O.Oracle(oracle_params(spec, ht=True))
+    x = np.indices(shape)[-1].ravel().astype(float)
+    T0, a, r0, b = 0.03, 2e-4, 0.1, 3e-3
+    rho = r0 + b * x
+    o.set("U", np.zeros((o.N, 3)))
+    o.set("rho", rho)
+    o.set("T", T0 + a * x)
+    o.call("heat_source")
+    lam = rho * cv * (1.0 / lam3 - 0.5) * fac
+    g = 0.5 * (1.0 / (r0 + b * (x + 1)) - 1.0 / (r0 + b * (x - 1)))
+    expect = (-lam * a * g / cv).reshape(shape)
+    ss = o.get("ss").reshape(shape)
+    inner = (slice(1, -1),) * len(shape)
+    assert np.allclose(ss[inner], expect[inner], rtol=1e-12, atol=0)
+    assert np.all(expect[inner] > 0)
