@@ -371,6 +371,10 @@ static void build_boundaries(orc_case* c) {
             break;
         }
         case ORC_PERIODIC_Y: {           /* periodicY.c:6-71 */
+            if ((int64_t)(nx - 1) * ny + (nx - 1) >= nxy) {
+                orc_fail(c, "Boundary list leaves the lattice for this box shape");
+                return;
+            }
             c->nbd = 4;
             int *x0 = bd_alloc(c, 0, ny - 2), *x1 = bd_alloc(c, 1, ny - 2);
             int *y0 = bd_alloc(c, 2, nx), *y1 = bd_alloc(c, 3, nx);
@@ -392,6 +396,20 @@ static void build_boundaries(orc_case* c) {
         default:
             orc_fail(c, "Unrecognized boundary type");
         }
+        /* the (nx-1)*ny indexing leaves the lattice when nx > ny: refused, as the
+         * reference would write past its arrays */
+        for (int b = 0; b < c->nbd; b++)
+            for (int i = 0; i < c->bdsize[b]; i++)
+                if (c->bd[b][i] < 0 || c->bd[b][i] >= nxy) {
+                    orc_fail(c, "Boundary list leaves the lattice for this box shape");
+                    for (int q = 0; q < c->nbd; q++) {
+                        free(c->bd[q]);
+                        c->bd[q] = NULL;
+                        c->bdsize[q] = 0;
+                    }
+                    c->nbd = 0;
+                    return;
+                }
         return;
     }
     switch (c->p.btype) {

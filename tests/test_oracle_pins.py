@@ -6,7 +6,7 @@ import pytest
 
 import _oracle as O
 from _cases import oracle_params
-from multiphaselb_amd.casefiles import CaseSpec
+from multiphaselb_amd.casefiles import BC, CaseSpec
 from multiphaselb_amd.initfields import maxwell, p_eos, vdw_critical
 
 LAM9 = (1.0, 1.1, 1.1, 1.0, 1.1, 1.0, 1.1, 0.8, 0.8)
@@ -258,3 +258,42 @@ def test_heat_source_conduction_term_closed_form(lbmodel):
     inner = (slice(1, -1),) * len(shape)
     assert np.allclose(ss[inner], expect[inner], rtol=1e-12, atol=0)
     assert np.all(expect[inner] > 0)
+
+
+def test_fixed_u_gives_the_wall_equilibrium_next_to_an_equilibrium_neighbour():
+    """fixedU rebuilds each missing population as f_eq(rho_nb, U_w) + (f_nb - f_eq(rho_nb,
+    u_nb)) (fixedU.c): with uniform density at rest (so the neighbour holds exactly its
+    equilibrium and no force acts) the missing populations of the wall nodes become the
+    textbook D2Q9 equilibrium w_k rho (1 + 3 c.U + 9/2 (c.U)^2 - 3/2 U^2) of the wall
+    velocity, and every other population stays as it was."""
+    Uw = (0.01, 0.002, 0.0)
+    spec = CaseSpec(lbmodel=0, nx=10, ny=10, btype=0, f_model=0, f_Lambda=LAM9, g_model=None,
+                    f_bc={"X0": BC("fixedU", Uw), "X1": BC("bounceBack"), "Y0": BC("bounceBack"),
+                          "Y1": BC("bounceBack")})
+    o = O.Oracle(oracle_params(spec, ht=False))
+    rho0 = 0.13
+    o.set("rho", np.full(o.N, rho0))
+    o.set("U", np.zeros((o.N, 3)))
+    o.set("T", np.full(o.N, 0.03))
+    o.init()
+    before = o.get("f").reshape(10, 10, 9).copy()
+    o.call("update_boundaries", 0)
+    after = o.get("f").reshape(10, 10, 9)
+    t = O.lattice_tables(O.D2Q9)
+    c, w = t["vel"][:, :2], t["omega"]
+    cu = c @ np.array(Uw[:2])
+    feq = w * rho0 * (1.0 + 3.0 * cu + 4.5 * cu ** 2 - 1.5 * np.dot(Uw, Uw))
+    missing = c[:, 0] == 1                          # pulled from x - c, outside at x = 0
+    for j in range(1, 9):                           # X0 nodes off the corners
+        np.testing.assert_allclose(after[j, 0, missing], feq[missing], rtol=1e-13, atol=0)
+        np.testing.assert_array_equal(after[j, 0, ~missing], before[j, 0, ~missing])
+    np.testing.assert_array_equal(after[1:-1, 1:-1], before[1:-1, 1:-1])
+
+
+def test_generic_box_lists_that_leave_the_lattice_are_refused():
+    """genericBoundary's (nx-1)*ny row index points past the lattice when nx > ny: the
+    oracle refuses the box, as the library does, instead of writing past its arrays."""
+    spec = CaseSpec(lbmodel=0, nx=10, ny=8, btype=0, f_model=0, f_Lambda=LAM9, g_model=None)
+    o = O.Oracle(oracle_params(spec, ht=False))
+    code, msg = o.error()
+    assert code != 0 and "leaves the lattice" in msg
