@@ -10,6 +10,16 @@ D3Q15 xuMRT scheme); --thermal adds the myMRT energy field with heat transfer.
 One "step" is one multiPhasePP time step over the whole lattice; the metric is MLUPS
 = stored nodes x steps / time / 1e6, timed with HIP events on the library's stream
 after W warm-up steps, inputs resident in HBM.  Prints one JSON line.
+
+Other workloads (--workload):
+  ensemble   1024 D2Q9 liMRT droplet/bubble cases run to steady state in one batch per
+             precision (FP64, FP32, FP16 storage): cases/s and Laplace-law errors vs FP64;
+  warmstart  learned warm start: solver-generated training profiles, GPU training of the
+             network, held-out D3Q15 128^3 bubbles run cold and warm to the same steady
+             criterion: time-to-steady ratio and final-state deviations;
+  accuracy   FP32 and FP16 storage against FP64 on the 256^3 bubble and the
+             128 x 128 x 384 thermal column: coexistence densities, Laplace jump,
+             spurious currents, temperature profile.
 """
 import argparse
 import json
