@@ -401,9 +401,15 @@ def run_warmstart(args):
     net = M.Mlp(W.WIDTHS, args.ws_batch)
     net.set_params(W.init_params(W.WIDTHS, 0))
     hist = W.train(net, X, Y, iters=args.ws_iters, seed=1)
+    # second network: the interface profiles alone (the first one decides flat or not)
+    keep = ~W.flat_targets(X, Y)
+    net_k = M.Mlp(W.WIDTHS, args.ws_batch)
+    net_k.set_params(W.init_params(W.WIDTHS, 1))
+    hist_k = W.train(net_k, X[keep], Y[keep], iters=args.ws_iters_kept, seed=2)
     t_train = time.perf_counter() - t_train
-    log("trained %d samples, %d params, %d iters in %.1f s, loss %.3e -> %.3e" % (
-        X.shape[0], net.P, args.ws_iters, t_train, hist[:20].mean(), hist[-50:].mean()))
+    log("trained %d samples (%d interface profiles), 2 x %d params, %d + %d iters in %.1f s, loss %.3e -> %.3e "
+        "(interfaces: %.3e -> %.3e)" % (X.shape[0], int(keep.sum()), net.P, args.ws_iters, args.ws_iters_kept, t_train,
+                                        hist[:20].mean(), hist[-50:].mean(), hist_k[:20].mean(), hist_k[-50:].mean()))
     evals = bubble3d_cases(args.ws_eval, 17, L3, args.ws_eval_droplets)
     inner, outer = r3 < 2.0, r3 > r3.max() - 2.0
     rows = []
@@ -417,10 +423,10 @@ def run_warmstart(args):
         # a proposal outside the physical range, or a warm run that diverges, falls back
         # to the cold start, whose time is then added to the warm total
         tw = time.perf_counter()
-        o = net.forward(W.features(c["tr"], c["radius"], L3, c["droplet"], 3)[None, :])[0, :W.NPROF]
-        prof = W.profile_density(W.profile_from_output(o, c["droplet"]), c["rho_v"], c["rho_l"])
+        prof, flat = W.propose(net, net_k, W.features(c["tr"], c["radius"], L3, c["droplet"], 3), c["droplet"],
+                               c["rho_v"], c["rho_l"])
         used = W.proposal_ok(prof, c["rho_v"], c["rho_l"])
-        flat = used and W.is_flat(prof, c["rho_v"], c["rho_l"])
+        flat = used and flat
         Rw = W.raster_case(case, prof, cen, mass, c["radius"], int(sites3.sum()), uniform=flat) if used else None
         if used:
             start(None)
@@ -479,7 +485,9 @@ def run_warmstart(args):
             "bubbles" if args.ws_eval_droplets == 0 else "droplets/bubbles", L3, L3 // 4),
                    "steady": "max|drho|/rho_l over 100 steps < %g" % thr, "max_steps": args.max_steps,
                    "train_2d": args.ws_2d, "train_3d": args.ws_3d, "loaded_samples": loaded, "eval": len(evals),
-                   "network": {"widths": W.WIDTHS, "params": net.P, "iters": args.ws_iters, "batch": args.ws_batch,
+                   "network": {"widths": W.WIDTHS, "params": net.P, "networks": 2,
+                               "roles": "all steady states (dissolves or not); interface profiles",
+                               "iters": [args.ws_iters, args.ws_iters_kept], "batch": args.ws_batch,
                                "3d_repeat": args.ws_3d_repeat}},
         "cases": {"eval": len(rows), "cold_unstable": len(rows) - len(pop), "compared": len(pop),
                   "cold_steady": int(sum(r["ok_cold"] for r in pop)), "warm_steady": int(sum(r["ok_warm"] for r in pop)),
@@ -656,7 +664,8 @@ def main():
     ap.add_argument("--ws-eval", type=int, default=256, help="warmstart: held-out 3D bubbles")
     ap.add_argument("--ws-size", type=int, default=128)
     ap.add_argument("--ws-eval-droplets", type=float, default=0.0, help="warmstart: share of droplets among the eval cases")
-    ap.add_argument("--ws-iters", type=int, default=4000)
+    ap.add_argument("--ws-iters", type=int, default=4000, help="warmstart: Adam iterations of the all-cases network")
+    ap.add_argument("--ws-iters-kept", type=int, default=16000, help="warmstart: iterations of the interface network")
     ap.add_argument("--ws-batch", type=int, default=256)
     ap.add_argument("--ws-3d-repeat", type=int, default=4, help="warmstart: times each 3D profile enters the training set")
     ap.add_argument("--ws-3d-offset", type=int, default=0, help="warmstart: first 3D training case index")

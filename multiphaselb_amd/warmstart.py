@@ -20,9 +20,10 @@ interpolated (monotone cubic) onto a uniform 0.05 grid; beyond |d| = 16 the end 
 continue.
 
 A droplet or bubble that dissolves in its closed box ends as a uniform field at the
-box's mean density; its target is that flat profile, so the network also learns which
-cases keep an interface.  A proposal outside the physical density range is not used
-(proposal_ok): the run then starts cold.
+box's mean density; its target is that flat profile, so a network trained on every
+steady state also learns which cases keep an interface.  The interface profile itself
+comes from a second network trained on the kept interfaces alone (propose).  A proposal
+outside the physical density range is not used (proposal_ok): the run then starts cold.
 """
 import numpy as np
 from scipy.interpolate import PchipInterpolator
@@ -203,6 +204,28 @@ def raster_case(case, prof, centre, mass, R0, nsites, uniform=False):
         fine = fine + (mass - case.set_radial_density(fine, FINE_D0, FINE_DD, centre, R)) / float(nsites)
     case.set_radial_density(fine, FINE_D0, FINE_DD, centre, R)
     return R
+
+
+def flat_targets(X, Y, tol=1e-3):
+    """Mask of the training samples whose target is a dissolved (flat) profile."""
+    droplet = np.asarray(X)[:, 6] > 0
+    cold = np.where(droplet[:, None], cold_profile(True)[None, :], cold_profile(False)[None, :])
+    y = cold + RES_SCALE * np.asarray(Y, np.float64)[:, :NPROF]
+    return np.ptp(y, axis=1) < tol
+
+
+def propose(net_all, net_kept, x, droplet, rho_v, rho_l):
+    """Density profile proposed for one case (features x): the network trained on every
+    steady state decides whether the case dissolves (a flat proposal, returned as is with
+    flat = True); otherwise the network trained on the interface profiles alone gives the
+    profile (its fit is not diluted by the flat targets).  Returns (profile, flat)."""
+    x = np.asarray(x, np.float32)[None, :]
+    prof = profile_density(profile_from_output(net_all.forward(x)[0, :NPROF], droplet), rho_v, rho_l)
+    if is_flat(prof, rho_v, rho_l):
+        return prof, True
+    if net_kept is None:
+        return prof, False
+    return profile_density(profile_from_output(net_kept.forward(x)[0, :NPROF], droplet), rho_v, rho_l), False
 
 
 def init_params(widths, seed):

@@ -91,3 +91,30 @@ def test_flat_proposals_are_recognised():
     assert W.is_flat(flat, RV, RL)
     step = W.profile_density(0.5 * (1.0 - np.tanh(2.0 * W.DK / 4.0)), RV, RL)
     assert not W.is_flat(step, RV, RL)
+
+
+class _Fixed:
+    """Stand-in network returning a fixed output row."""
+
+    def __init__(self, o):
+        self.o = np.asarray(o, np.float32)
+
+    def forward(self, x):
+        return np.tile(self.o, (len(x), 1))
+
+
+def test_flat_targets_and_two_network_proposal():
+    y_flat = np.full(W.NPROF, 0.9)
+    y_step = 0.5 * (1.0 + np.tanh(2.0 * W.DK / 4.0))
+    x_b = W.features(0.8, 20.0, 128, False, 3)
+    X = np.stack([x_b, x_b])
+    Y = np.stack([np.concatenate([W.target_from_profile(y, False), np.zeros(W.NPROF)]) for y in (y_flat, y_step)])
+    np.testing.assert_array_equal(W.flat_targets(X, Y), [True, False])
+    out = lambda y: np.concatenate([W.target_from_profile(y, False), np.zeros(W.NPROF)])
+    # the first network says "dissolves": its flat profile is the proposal
+    prof, flat = W.propose(_Fixed(out(y_flat)), _Fixed(out(y_step)), x_b, False, RV, RL)
+    assert flat and np.allclose(prof, W.profile_density(y_flat, RV, RL), atol=1e-6)
+    # otherwise the interface network's profile is used
+    y_other = 0.5 * (1.0 + np.tanh(2.0 * W.DK / 6.0))
+    prof, flat = W.propose(_Fixed(out(y_step)), _Fixed(out(y_other)), x_b, False, RV, RL)
+    assert not flat and np.allclose(prof, W.profile_density(y_other, RV, RL), atol=1e-6)
