@@ -264,14 +264,14 @@ def run_to_steady_case(case, rho_l, thr, check_every, max_steps):
     return steps, time.perf_counter() - t0, False
 
 
-def bubble3d_cases(n, seed, L, droplet_share):
+def bubble3d_cases(n, seed, L, droplet_share, rmin=6.0):
     from multiphaselb_amd.initfields import splitmix64, vdw_critical, maxwell
 
     u = splitmix64(seed, 3 * n).reshape(n, 3)
     tc = vdw_critical(0.5, 4.0)[1]
     out = []
     for k in range(n):
-        R = 6.0 + u[k, 0] * (L / 4.0 - 6.0)
+        R = rmin + u[k, 0] * (L / 4.0 - rmin)
         tr = 0.6 + 0.35 * u[k, 1]
         rv, rl = maxwell(tr * tc, a=0.5, b=4.0)
         out.append(dict(L=L, radius=R, tr=tr, T=tr * tc, droplet=bool(u[k, 2] < droplet_share), rho_v=rv, rho_l=rl,
@@ -349,7 +349,12 @@ def run_warmstart(args):
         case.init()
 
     t3 = time.perf_counter()
-    train3 = bubble3d_cases(args.ws_3d_offset + args.ws_3d, 13, L3, 0.5)[args.ws_3d_offset:]
+    # training cases: seed 13 over the evaluation's ranges; --ws-3d-rmin > 6 draws another
+    # set (seed 19, bubbles only) from the larger radii, where interfaces survive
+    if args.ws_3d_rmin > 6.0:
+        train3 = bubble3d_cases(args.ws_3d_offset + args.ws_3d, 19, L3, 0.0, args.ws_3d_rmin)[args.ws_3d_offset:]
+    else:
+        train3 = bubble3d_cases(args.ws_3d_offset + args.ws_3d, 13, L3, 0.5)[args.ws_3d_offset:]
     kept3 = flat3 = moving3 = 0
     for i, c in enumerate(train3):
         start(cold_field(c))
@@ -669,6 +674,8 @@ def main():
     ap.add_argument("--ws-batch", type=int, default=256)
     ap.add_argument("--ws-3d-repeat", type=int, default=4, help="warmstart: times each 3D profile enters the training set")
     ap.add_argument("--ws-3d-offset", type=int, default=0, help="warmstart: first 3D training case index")
+    ap.add_argument("--ws-3d-rmin", type=float, default=6.0,
+                    help="warmstart: smallest training radius (> 6: a separate set of larger bubbles)")
     ap.add_argument("--ws-save-data", default="", help="warmstart: save the training profiles (npz)")
     ap.add_argument("--ws-load-data", default="", help="warmstart: add training profiles from npz files")
     ap.add_argument("--cases", type=int, default=1024, help="ensemble: number of cases")
