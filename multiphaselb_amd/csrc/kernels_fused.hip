@@ -582,8 +582,8 @@ __global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
 // workgroups that own those sites run on the same XCD at the same time (work items are
 // handed out so that each XCD takes a contiguous range of chunk-major items).  The
 // column is 64 x 4 (256-byte rows per population plane): on the 256^3 bubble it runs 6%
-// faster than 32 x 8 and 48% faster than 16 x 16, although its ring (140 sites per 256)
-// is the largest of the three.  The ring needs MRING <= 256 threads.
+// faster than 32 x 8, 48% faster than 16 x 16 and 6% faster than 128 x 2.  One thread
+// per ring site: MRING <= 256.
 constexpr int MX = 64, MY = 4, MHX = MX + 2, MHY = MY + 2, MSLOT = MHX * MHY;
 constexpr int MRING = 2 * MHX + 2 * MY;
 static_assert(MX * MY == 256 && MRING <= 256, "one thread per column site and per ring site");
