@@ -69,6 +69,13 @@ struct FArgs {
     float* Unew;
     float Tconst;
     int Tuniform;
+    // loop-invariant factors of the FP32 arithmetic, set on the host: the surface-tension
+    // source coefficients 12 sigma / (1/L1 - 1/2), 12 sigma / (1/L2 - 1/2) (D2Q9) or
+    // 6 sigma / (1/L1 - 1/2) (D3Q15), and for the van der Waals potential R, b, a / M^2,
+    // 1 / G and M (vdw = 1)
+    float sf1, sf2;
+    int vdw;
+    float vR, vb, va_m2, vinv_g, vM;
     // optional outputs (materialisation)
     float* out_rho;
     float* out_U;
@@ -113,6 +120,17 @@ struct NodePos {
     bool valid;
     unsigned X[3], Y[3], Z[3];   // wrapped coordinates u-1, u, u+1 premultiplied by the strides
 };
+
+// psi = potential(rho, T) (potential.c:7-13) in FP32 with the loop-invariant factors taken
+// from the host for the van der Waals EOS (one division instead of three); other EOS
+// types use the shared form.
+__device__ inline float fpot(const FArgs& a, float rho, float T) {
+    if (a.vdw) {
+        const float p = rho * a.vR * T / (a.vM - rho * a.vb) - a.va_m2 * rho * rho;
+        return sqrtf(2.f * (p - rho * float(kCs2)) * a.vinv_g);
+    }
+    return potential<float>(a.eos, rho, T);
+}
 
 // Block b covers the 256 consecutive node ids [256 b, 256 b + 256).
 template <class L>
@@ -255,7 +273,7 @@ __global__ void __launch_bounds__(256) k_fused_density(FArgs a) {
         a.sumg[p.id] = sg;
     } else {
         const float T = a.Tuniform ? a.Tconst : a.Told[p.id];
-        a.psi[p.id] = potential<float>(a.eos, rho, T);
+        a.psi[p.id] = fpot(a, rho, T);
         if (a.out_rho) a.out_rho[p.id] = rho;
     }
 }
@@ -318,7 +336,7 @@ __global__ void __launch_bounds__(256) k_fused_heat(FArgs a) {
     }
     a.S[id] = Sv;
     a.Tnew[id] = T;
-    a.psi[id] = potential<float>(a.eos, rho, T);
+    a.psi[id] = fpot(a, rho, T);
 }
 
 // Interaction force (interactionForce.c:47-79), velocity (liMRTVelocity.c /
@@ -359,6 +377,7 @@ __device__ inline void collide_node(const FArgs& a, float* v, float rho, const f
         Ft[d] = Fi[d] + grav + (interior ? (float)a.eos.fext[d] : 0.f);
         Fu[d] = a.fmodel == 0 ? Ft[d] : Fi[d] + grav;
     }
+    const float irho = 1.f / rho;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         float j = 0.f;
@@ -368,7 +387,7 @@ __device__ inline void collide_node(const FArgs& a, float* v, float rho, const f
             if (c == 1) j += v[P.q[k]] - v[P.r[k]];
             else if (c == -1) j -= v[P.q[k]] - v[P.r[k]];
         }
-        U[d] = (j + 0.5f * Fu[d]) / rho;
+        U[d] = (j + 0.5f * Fu[d]) * irho;
     }
     // MRT collision of f (liMRTCollision.c:33-106 / xuMRTCollision.c:42-92)
     float m[L::Q], meq[L::Q], Sf[L::Q];
@@ -388,8 +407,9 @@ __device__ inline void collide_node(const FArgs& a, float* v, float rho, const f
         const float uF = U[0] * Fu[0] + U[1] * Fu[1];
         const float fi2 = Fi[0] * Fi[0] + Fi[1] * Fi[1];
         Sf[0] = 0.f;
-        Sf[1] = 6.f * uF + 12.f * a.sigma * fi2 / (psi2 * ((1.f / a.fl[1]) - 0.5f));
-        Sf[2] = -6.f * uF - 12.f * a.sigma * fi2 / (psi2 * ((1.f / a.fl[2]) - 0.5f));
+        const float fp = fi2 / psi2;
+        Sf[1] = 6.f * uF + a.sf1 * fp;
+        Sf[2] = -6.f * uF - a.sf2 * fp;
         Sf[3] = Fu[0];
         Sf[4] = -Fu[0];
         Sf[5] = Fu[1];
@@ -415,7 +435,7 @@ __device__ inline void collide_node(const FArgs& a, float* v, float rho, const f
         const float uF = U[0] * Fu[0] + U[1] * Fu[1] + U[2] * Fu[2];
         const float F2 = Fu[0] * Fu[0] + Fu[1] * Fu[1] + Fu[2] * Fu[2];
         Sf[0] = 0.f;
-        Sf[1] = 2.f * uF + 6.f * a.sigma * F2 / (psi2 * ((1.f / a.fl[1]) - 0.5f));
+        Sf[1] = 2.f * uF + a.sf1 * F2 / psi2;
         Sf[2] = -10.f * uF;
         Sf[3] = Fu[0];
         Sf[4] = (-7.f / 3.f) * Fu[0];
@@ -610,7 +630,7 @@ template <class L, typename S>
 __device__ inline float march_psi(const FArgs& a, int x, int y, int z, float* v) {
     const float rho = march_pull<L, S>(a, x, y, z, v);
     const float T = a.Tuniform ? a.Tconst : a.Told[(int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy];
-    return potential<float>(a.eos, rho, T);
+    return fpot(a, rho, T);
 }
 
 // The populations pulled for psi at plane z + 1 stay in registers (fn) until plane z + 1
@@ -793,6 +813,18 @@ FArgs make_args(mplb_case* c) {
     a.sigma = (float)fp.sigma;
     a.kappa = (float)fp.kappa;
     for (int q = 0; q < 15; q++) a.fl[q] = (float)fp.Lambda[q];
+    if (m.lbmodel == 0) {
+        a.sf1 = (float)(12.0 * fp.sigma / (1.0 / fp.Lambda[1] - 0.5));
+        a.sf2 = (float)(12.0 * fp.sigma / (1.0 / fp.Lambda[2] - 0.5));
+    } else {
+        a.sf1 = (float)(6.0 * fp.sigma / (1.0 / fp.Lambda[1] - 0.5));
+    }
+    a.vdw = c->eos.type == EOS_VDW && c->eos.G != 0.0 ? 1 : 0;
+    a.vR = (float)c->eos.R;
+    a.vb = (float)c->eos.b;
+    a.va_m2 = (float)(c->eos.a / (c->eos.M * c->eos.M));
+    a.vinv_g = c->eos.G != 0.0 ? (float)(1.0 / c->eos.G) : 0.f;
+    a.vM = (float)c->eos.M;
     if (c->fc[1].present) {
         const ModelParams& gp = c->fc[1].mp;
         a.gmodel = gp.model == M_MYMRT ? 1 : 3;
