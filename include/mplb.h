@@ -132,19 +132,26 @@ int mplb_glibc_rand(unsigned seed, int n, int32_t* out);
  * tables, as the solver's readLatticeMesh.c does; otherwise the box is built from
  * properties/latticeProperties. */
 /* processor0/lattice/{points,ghosts,neighbours,recvGhosts,sendGhosts,boundary,vtkCells}
- * under dir (IO/writeLatticeMesh.c:20-226, one rank). */
+ * under dir (IO/writeLatticeMesh.c:20-226, one rank); with dataFormat ensight also
+ * dir/lattice.geo and dir/lbm.case first (writeLatticeMesh.c:17-22,
+ * IO/ensight/writeMeshToEnsight.c, updateCaseFile.c). */
 int mplb_write_lattice(const mplb_case* c, const char* dir);
 /* rho, T, p, U, f, g of the current time level into dir/processor0/<time>/ in the case's
  * dataFormat (multiPhasePP.c:276-309 with writeScalarField.c / writeVectorField.c,
  * including their NaN check: MPLB_E_NAN): asciiRaw or binaryRaw, one file per field; or
  * pvtu (IO/pvtu/writeMeshToPvtu.c, writeScalarToPvtu.c, writeArrayToPvtu.c,
  * writePvtuExtra.c, writeMainPvd.c): processor0/<time>/fields.vtu, <time>/fields.pvtu and
- * case.pvd under dir, with the field names of start/initialFields. */
+ * case.pvd under dir, with the field names of start/initialFields; or binary EnSight Gold
+ * (multiPhasePP.c:283-285 updateCaseFile, IO/ensight/write{Scalar,Vector,LbeField}To
+ * Ensight.c): the current time appended to dir/lbm.case, then dir/lattice.<name>_<index>
+ * per field (f and g: lattice.f<k>_<index> per population), index = the time's position
+ * in the case file's list. */
 int mplb_write_fields(mplb_case* c, const char* dir);
 /* Field <name> of time level <time> under dir, comps values per node, in the case's
  * dataFormat; asciiRaw/binaryRaw values pass through float as the reference's readers do
  * (readScalarFromAsciiRaw.c); pvtu values are read in double (readScalarFromPvtu.c /
- * readArrayFromPvtu.c). */
+ * readArrayFromPvtu.c); ensight files are float32, found through lbm.case's time list
+ * (IO/ensight/read*FromEnsight.c, timeToIndex.c: index 0 for a time not listed). */
 int mplb_read_field(const mplb_case* c, const char* dir, int time, const char* name, int comps, double* out,
                     size_t n);
 

@@ -8,7 +8,9 @@
 //     in readScalarFromAsciiRaw.c / readArrayFromAsciiRaw.c;
 //   * the pvtu format (IO/pvtu/*): processor0/<time>/fields.vtu with the points, the
 //     VTK cells and every field as an ascii Float32 DataArray, <time>/fields.pvtu
-//     naming it, and case.pvd listing the time folders present.
+//     naming it, and case.pvd listing the time folders present;
+//   * binary EnSight Gold (IO/ensight/*): lattice.geo, the lbm.case time list and one
+//     float32 file per variable and time index in the case folder.
 #include <hip/hip_runtime.h>
 
 #include <dirent.h>
@@ -19,6 +21,7 @@
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -126,19 +129,25 @@ std::string joined(const std::vector<std::string>& v, bool trailing) {
     return s;
 }
 
+// readVTKInfo.c: the scalar, vector and pdf field names listed in start/initialFields
+// (empty lists when the file is absent)
+void field_names(const mplb_case* c, std::vector<std::string>* sc, std::vector<std::string>* vc,
+                 std::vector<std::string>* pc) {
+    Dict init;
+    if (init.open(c->dir + "/start/initialFields")) {
+        init.string_list("scalarFields", sc);
+        init.string_list("vectorFields", vc);
+        init.string_list("pdfFields", pc);
+    }
+}
+
 // IO/writeMeshToVTK.c + writeScalarToPvtu.c + writeArrayToPvtu.c + writePvtuExtra.c +
 // writeMainPvd.c for one rank
 int write_pvtu(mplb_case* c, const std::string& dir, const FieldItem* items, int nitems) {
     const Mesh& m = c->mesh;
     const int64_t N = m.N;
-    // readVTKInfo.c: the names listed in start/initialFields
     std::vector<std::string> sc, vc, pc;
-    Dict init;
-    if (init.open(c->dir + "/start/initialFields")) {
-        init.string_list("scalarFields", &sc);
-        init.string_list("vectorFields", &vc);
-        init.string_list("pdfFields", &pc);
-    }
+    field_names(c, &sc, &vc, &pc);
     std::vector<long long> conn;
     int ctype = 0;
     if (!vtk_cells(c, &conn, &ctype)) return ffail(c, MPLB_E_IO, "Ill defined vtkCells file");
@@ -278,6 +287,201 @@ int read_pvtu(mplb_case* c, const std::string& path, const char* name, int comps
     }
 }
 
+// ---- EnSight Gold, binary (IO/ensight/*, one rank: part 1) ------------------------
+// Every file sits in the case folder: lattice.geo, lbm.case and one file per variable
+// and time index, lattice.<name>_<index> (population k of an lbe field: lattice.<name>k_
+// <index>).  Text records are 80-byte NUL-padded blocks, numbers int32 / float32.
+
+bool put80(FILE* f, const char* s) {
+    char b[80];
+    std::memset(b, 0, sizeof b);
+    std::strncpy(b, s, sizeof b - 1);
+    return std::fwrite(b, 1, sizeof b, f) == sizeof b;
+}
+
+bool put_i32(FILE* f, int32_t v) { return std::fwrite(&v, sizeof v, 1, f) == 1; }
+
+// the time list of <dir>/lbm.case as timeToIndex.c and updateCaseFile.c scan it: the
+// integer after the words "number of steps:", then that many integers after "time
+// values:".  false when the file is absent.
+bool case_times(const std::string& dir, std::vector<long long>* t) {
+    t->clear();
+    File f(dir + "/lbm.case", "r");
+    if (!f.f) return false;
+    std::vector<std::string> w;
+    char b[256];
+    while (std::fscanf(f.f, "%255s", b) == 1) w.push_back(b);
+    long long ns = -1;
+    for (size_t i = 0; i + 3 < w.size() && ns < 0; i++)
+        if (w[i] == "number" && w[i + 1] == "of" && w[i + 2] == "steps:") ns = std::atoll(w[i + 3].c_str());
+    for (size_t i = 0; i + 1 < w.size() && ns > 0; i++)
+        if (w[i] == "time" && w[i + 1] == "values:") {
+            for (long long k = 0; k < ns && i + 2 + (size_t)k < w.size(); k++)
+                t->push_back(std::atoll(w[i + 2 + (size_t)k].c_str()));
+            break;
+        }
+    return true;
+}
+
+// timeToIndex.c: position of `time` in the case file's list (the last match), 0 when it is
+// not listed or there is no case file
+long long ensight_index(const std::string& dir, long long time) {
+    std::vector<long long> t;
+    case_times(dir, &t);
+    long long idx = 0;
+    for (size_t k = 0; k < t.size(); k++)
+        if (t[k] == time) idx = (long long)k;
+    return idx;
+}
+
+// updateCaseFile.c: append the current time to the case file's list unless it is already
+// the last entry; a case folder without lbm.case starts the list at time 0 (whatever the
+// current time: the field files of that write then carry index 0).  The VARIABLE block
+// names the fields of start/initialFields, pdf fields one scalar per population.
+int update_case_file(mplb_case* c, const std::string& dir) {
+    std::vector<long long> t;
+    if (case_times(dir, &t)) {
+        if (t.empty() || t.back() != c->current) t.push_back(c->current);
+    } else {
+        t.assign(1, 0);
+    }
+    std::vector<std::string> sc, vc, pc;
+    field_names(c, &sc, &vc, &pc);
+    File f(dir + "/lbm.case", "w");
+    if (!f.f) return ffail(c, MPLB_E_IO, "Unable to write lbm.case");
+    // the value column starts at character 27 on every keyword line
+    std::fprintf(f.f, "#\n# EnSight 7.4.1 ((n))\n# Case File: lattice.case\n\nFORMAT\n\n");
+    std::fprintf(f.f, "type:  ensight gold\n\nGEOMETRY\n\n%-27s%s\n\nVARIABLE\n\n", "model:", "lattice.geo");
+    for (const std::string& n : sc) std::fprintf(f.f, "%-27s%s lattice.%s_*\n", "scalar per node:", n.c_str(), n.c_str());
+    for (const std::string& n : pc)
+        for (int k = 0; k < c->mesh.Q; k++)
+            std::fprintf(f.f, "%-27s%s%d lattice.%s%d_*\n", "scalar per node:", n.c_str(), k, n.c_str(), k);
+    for (const std::string& n : vc) std::fprintf(f.f, "%-27s%s lattice.%s_*\n", "vector per node:", n.c_str(), n.c_str());
+    std::fprintf(f.f, "\nTIME\n%-27s%d\n%-27s%d\n%-27s%d\n%-27s%d\n", "time set:", 1, "number of steps:", (int)t.size(),
+                 "filename start number:", 0, "filename increment:", 1);
+    for (size_t k = 0; k < t.size(); k++) std::fprintf(f.f, "%-27s%lld\n", k ? "" : "time values:", t[k]);
+    if (std::ferror(f.f)) return ffail(c, MPLB_E_IO, "Unable to write lbm.case");
+    return 0;
+}
+
+// writeMeshToEnsight.c: lattice.geo with the node coordinates (x block, y block, z block)
+// and the VTK cells as quad4 (D2Q9) or hexa8 (D3Q15), 1-based, each face's last two
+// nodes swapped (VTK's lexicographic voxel order -> EnSight's cyclic order)
+int write_ensight_geo(mplb_case* c, const std::string& dir) {
+    const Mesh& m = c->mesh;
+    std::vector<long long> conn;
+    int ctype = 0;
+    if (!vtk_cells(c, &conn, &ctype)) return ffail(c, MPLB_E_IO, "Ill defined vtkCells file");
+    if (ctype != (m.d == 3 ? 8 : 4)) return ffail(c, MPLB_E_IO, "vtkCells do not hold one voxel per lattice cell");
+    {
+        File f(dir + "/lattice.geo", "wb");
+        if (!f.f) return ffail(c, MPLB_E_IO, "Unable to write lattice.geo");
+        for (const char* h : {"C Binary", "EnSight Model Geometry File", "EnSight 7.4.1", "node id assign",
+                              "element id assign"})
+            put80(f.f, h);
+    }
+    const int rc = update_case_file(c, dir);
+    if (rc) return rc;
+    File f(dir + "/lattice.geo", "ab");
+    if (!f.f) return ffail(c, MPLB_E_IO, "Unable to write lattice.geo");
+    bool ok = put80(f.f, "part") && put_i32(f.f, 1) && put80(f.f, "Point field") && put80(f.f, "coordinates") &&
+              put_i32(f.f, (int32_t)m.N);
+    std::vector<float> xyz((size_t)m.N);
+    for (int d = 0; d < 3 && ok; d++) {
+        for (int64_t i = 0; i < m.N; i++) {
+            int p[3];
+            if (!m.points.empty()) {
+                p[0] = m.points[3 * i];
+                p[1] = m.points[3 * i + 1];
+                p[2] = m.points[3 * i + 2];
+            } else {
+                coords(m, i, p);
+            }
+            xyz[(size_t)i] = (float)p[d];
+        }
+        ok = std::fwrite(xyz.data(), sizeof(float), xyz.size(), f.f) == xyz.size();
+    }
+    const long long ncells = (long long)(conn.size() / (size_t)ctype);
+    ok = ok && put80(f.f, m.d == 3 ? "hexa8" : "quad4") && put_i32(f.f, (int32_t)ncells);
+    static const int order[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    std::vector<int32_t> ids((size_t)conn.size());
+    for (long long i = 0; i < ncells; i++)
+        for (int j = 0; j < ctype; j++) ids[(size_t)(i * ctype + j)] = (int32_t)(conn[(size_t)(i * ctype + order[j])] + 1);
+    ok = ok && std::fwrite(ids.data(), sizeof(int32_t), ids.size(), f.f) == ids.size();
+    if (!ok) return ffail(c, MPLB_E_IO, "Unable to write lattice.geo");
+    return 0;
+}
+
+// writeScalarToEnsight.c / writeVectorToEnsight.c / writeLbeFieldToEnsight.c body: the
+// variable's name, "part", 1, "coordinates", then the values as float32
+bool ensight_var(const std::string& path, const char* header, const float* v, size_t n) {
+    File f(path, "wb");
+    return f.f && put80(f.f, header) && put80(f.f, "part") && put_i32(f.f, 1) && put80(f.f, "coordinates") &&
+           std::fwrite(v, sizeof(float), n, f.f) == n;
+}
+
+// multiPhasePP.c:283-301 with dataFormat ensight: the case file first, then each field
+// (NaN check, then its file): scalars, U as three component blocks, f and g one file per
+// population, each headed by the field's own name
+int write_ensight(mplb_case* c, const std::string& dir, const FieldItem* items, int nitems) {
+    if (!mkdirs(dir)) return ffail(c, MPLB_E_IO, "Unable to create " + dir);
+    int rc = update_case_file(c, dir);
+    if (rc) return rc;
+    const long long idx = ensight_index(dir, c->current);
+    const int64_t N = c->mesh.N;
+    std::vector<double> h;
+    std::vector<float> v((size_t)N * 3);
+    for (int k = 0; k < nitems; k++) {
+        const FieldItem& it = items[k];
+        rc = fetch_checked(c, it, &h);
+        if (rc) return rc;
+        const std::string tail = "_" + std::to_string(idx);
+        if (it.comps == 1 || it.comps == 3) {
+            for (int j = 0; j < it.comps; j++)
+                for (int64_t i = 0; i < N; i++) v[(size_t)(j * N + i)] = (float)h[(size_t)(i * it.comps + j)];
+            if (!ensight_var(dir + "/lattice." + it.name + tail, it.name, v.data(), (size_t)(N * it.comps)))
+                return ffail(c, MPLB_E_IO, std::string("Unable to write lattice.") + it.name + tail);
+        } else {
+            for (int q = 0; q < it.comps; q++) {
+                for (int64_t i = 0; i < N; i++) v[(size_t)i] = (float)h[(size_t)(i * it.comps + q)];
+                const std::string path = dir + "/lattice." + it.name + std::to_string(q) + tail;
+                if (!ensight_var(path, it.name, v.data(), (size_t)N)) return ffail(c, MPLB_E_IO, "Unable to write " + path);
+            }
+        }
+    }
+    return 0;
+}
+
+// readScalarFromEnsight.c / readVectorFromEnsight.c / readLbeFieldFromEnsight.c: the
+// file of the case-file index of `time`, values after the 244-byte header (three text
+// records and the part number), through float
+int read_ensight(mplb_case* c, const std::string& dir, int time, const char* name, int comps, double* out) {
+    const int64_t N = c->mesh.N;
+    const std::string tail = "_" + std::to_string(ensight_index(dir, time));
+    const long header = 3 * 80 + (long)sizeof(int32_t);
+    std::vector<float> v((size_t)N * (comps == 3 ? 3 : 1));
+    auto load = [&](const std::string& path) -> int {
+        File f(path, "rb");
+        if (!f.f) return ffail(c, MPLB_E_IO, "Unable to open " + path);
+        if (std::fseek(f.f, header, SEEK_SET) != 0 || std::fread(v.data(), sizeof(float), v.size(), f.f) != v.size())
+            return ffail(c, MPLB_E_IO, "Short file " + path);
+        return 0;
+    };
+    if (comps == 1 || comps == 3) {
+        const int rc = load(dir + "/lattice." + name + tail);
+        if (rc) return rc;
+        for (int j = 0; j < comps; j++)
+            for (int64_t i = 0; i < N; i++) out[(size_t)(i * comps + j)] = (double)v[(size_t)(j * N + i)];
+        return 0;
+    }
+    for (int q = 0; q < comps; q++) {
+        const int rc = load(dir + "/lattice." + name + std::to_string(q) + tail);
+        if (rc) return rc;
+        for (int64_t i = 0; i < N; i++) out[(size_t)(i * comps + q)] = (double)v[(size_t)i];
+    }
+    return 0;
+}
+
 }  // namespace
 
 namespace mplb {
@@ -372,6 +576,11 @@ int mplb_write_lattice(const mplb_case* cc, const char* dir) {
     const Mesh& m = c->mesh;
     const std::string base = std::string(dir) + "/processor0/lattice/";
     if (!mkdirs(base)) return ffail(c, MPLB_E_IO, "Unable to create " + base);
+    // writeLatticeMesh.c:17-22: with dataFormat ensight the geometry and case file first
+    if (c->data_format == "ensight") {
+        const int rc = write_ensight_geo(c, dir);
+        if (rc) return rc;
+    }
     const int64_t N = m.N;
     {
         File f(base + "points", "w");
@@ -440,6 +649,7 @@ int mplb_write_fields(mplb_case* c, const char* dir) {
     const FieldItem items[] = {{"rho", 1, false}, {"T", 1, false}, {"p", 1, false},
                                {"U", 3, true},    {"f", c->mesh.Q, true}, {"g", c->mesh.Q, true}};
     if (c->data_format == "pvtu") return write_pvtu(c, dir, items, 6);
+    if (c->data_format == "ensight") return write_ensight(c, dir, items, 6);
     const bool binary = c->data_format == "binaryRaw";
     if (!binary && c->data_format != "asciiRaw")
         return ffail(c, MPLB_E_MODEL, "dataFormat " + c->data_format + " is not written by this library");
@@ -476,6 +686,7 @@ int mplb_read_field(const mplb_case* cc, const char* dir, int time, const char* 
     if ((int64_t)n != c->mesh.N * comps) return ffail(c, MPLB_E_ARG, "output size differs from nodes x components");
     if (c->data_format == "pvtu")
         return read_pvtu(c, std::string(dir) + "/processor0/" + std::to_string(time) + "/fields.vtu", name, comps, out, n);
+    if (c->data_format == "ensight") return read_ensight(c, dir, time, name, comps, out);
     const bool binary = c->data_format == "binaryRaw";
     if (!binary && c->data_format != "asciiRaw")
         return ffail(c, MPLB_E_MODEL, "dataFormat " + c->data_format + " is not read by this library");

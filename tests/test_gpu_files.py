@@ -98,3 +98,49 @@ def test_pvtu_fields(name):
     c.write_fields(d)
     pvd = open(os.path.join(d, "case.pvd")).read()
     assert pvd.count("<DataSet") == 2 and 'file="%d/fields.pvtu"' % t in pvd
+
+
+@pytest.mark.parametrize("name", ["d2q9_bubble_thermal", "d3q15_bubble_nht"])
+def test_ensight_fields(name):
+    """ensight output (multiPhasePP.c:283-301 with IO/ensight/*): each write step appends
+    its time to lbm.case and writes lattice.<field>_<index> files (U as three component
+    blocks, f and g one file per population) that read back as the float32 values; a
+    folder without lbm.case starts its list at time 0, as updateCaseFile.c does."""
+    import _ensight as E
+    spec, fk, ht, frozen, steps = CASES[name]
+    spec = dataclasses.replace(spec, data_format="ensight")
+    d = make_case_dir(spec)
+    c = M.Case(d, precision=M.FP64, heat_transfer=ht)
+    rho, U, T = sphere_fields(spec, seed=6, **fk)
+    for n, v in (("rho", rho), ("U", U), ("T", T)):
+        c.set(n, v)
+    c.init()
+    c.write_lattice(d)                             # lattice.geo + lbm.case listing time 0
+    Q = c.Q
+    fields = (("rho", 1), ("T", 1), ("p", 1), ("U", 3), ("f", Q), ("g", Q))
+    saved, times = {}, [0]
+    for w in range(2):
+        c.step(spec.write_interval)
+        t = c.get_info().current_time
+        times.append(t)
+        c.write_fields(d)
+        assert E.case_times(os.path.join(d, "lbm.case")) == times
+        saved[t] = {n: c.get(n).astype(np.float32).astype(np.float64) for n, _ in fields}
+        print("write", w, "time", t, flush=True)
+    for t, want in saved.items():
+        for n, comps in fields:
+            np.testing.assert_array_equal(c.read_field(d, t, n, comps), want[n].reshape(-1) if comps == 1 else want[n])
+    t = times[-1]
+    idx = len(times) - 1
+    hdr, part, v = E.read_var(os.path.join(d, "lattice.U_%d" % idx), 3 * c.N)
+    assert (hdr, part) == ("U", 1)
+    np.testing.assert_array_equal(v.reshape(3, c.N).T, saved[t]["U"])
+    hdr, part, v = E.read_var(os.path.join(d, "lattice.f%d_%d" % (Q - 1, idx)), c.N)
+    assert hdr == "f"
+    np.testing.assert_array_equal(v, saved[t]["f"][:, Q - 1])
+    # a fresh folder: no case file, so the list is [0] and the files carry index 0
+    d2 = make_case_dir(spec)
+    c.write_fields(d2)
+    assert E.case_times(os.path.join(d2, "lbm.case")) == [0]
+    np.testing.assert_array_equal(c.read_field(d2, t, "rho", 1), saved[t]["rho"])
+    assert os.path.exists(os.path.join(d2, "lattice.g0_0"))
