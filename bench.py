@@ -118,7 +118,7 @@ def cpu_baseline(args, budget_s=12.0):
     dt = time.perf_counter() - t0
     return dict(value=round(N * steps / dt / 1e6, 3), unit="MLUPS", cores=threads, kind="port",
                 sample="D3Q15 %s bubble 64^3 FP64 oracle, %d steps in %.1f s" %
-                       ("xuMRT+myMRT thermal" if args.thermal else "xuMRT", steps, dt))
+                       ("liMRT+myMRT thermal" if args.thermal else "liMRT", steps, dt))
 
 
 def laplace(rho, p, L, rho_v, rho_l):
