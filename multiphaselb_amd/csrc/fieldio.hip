@@ -19,6 +19,8 @@
 #include <algorithm>
 
 #include <cerrno>
+#include <climits>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -373,6 +375,8 @@ int write_ensight_geo(mplb_case* c, const std::string& dir) {
     int ctype = 0;
     if (!vtk_cells(c, &conn, &ctype)) return ffail(c, MPLB_E_IO, "Ill defined vtkCells file");
     if (ctype != (m.d == 3 ? 8 : 4)) return ffail(c, MPLB_E_IO, "vtkCells do not hold one voxel per lattice cell");
+    if (m.N > INT32_MAX || (int64_t)(conn.size() / (size_t)ctype) > INT32_MAX)
+        return ffail(c, MPLB_E_IO, "lattice.geo counts nodes and cells in 32-bit integers: lattice too large");
     {
         File f(dir + "/lattice.geo", "wb");
         if (!f.f) return ffail(c, MPLB_E_IO, "Unable to write lattice.geo");

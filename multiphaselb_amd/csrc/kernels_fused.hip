@@ -1073,6 +1073,8 @@ bool fused_supported(const mplb_case* c, std::string* why) {
         return false;
     };
     if (c->opts.precision == MPLB_FP64) return no("the fused kernels compute in FP32");
+    // node ids are 32-bit in the fused kernels (workgroup * 256 + lane)
+    if (c->mesh.N > (int64_t(1) << 31) - 256) return no("more than 2^31 - 256 nodes");
     const Mesh& m = c->mesh;
     if (!m.structured) return no("lattice nodes not in structured box order");
     if (m.nz > 1 && m.d == 2) return no("2D box with nz > 1");
