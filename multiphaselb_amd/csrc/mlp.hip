@@ -165,13 +165,28 @@ __global__ void k_loss_final(const float* __restrict__ rowsum, int B, int n, con
     if (hist) hist[*counter] = L;
 }
 
-// db[c] = sum over the batch of dz[b][c], in batch order
-__global__ void k_bias_grad(const float* __restrict__ dz, int B, int n, float* db) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    float s = 0.f;
-    for (int b = 0; b < B; b++) s += dz[(size_t)b * n + c];
-    db[c] = s;
+// db[c] = sum over the batch of dz[b][c].  A 256-lane workgroup takes 64 columns: lane
+// (c, k) sums rows k, k + 4, k + 8, ... with four independent accumulators (loads in
+// flight instead of one dependent chain), and the four slices of a column are added in
+// slice order through LDS, so the result does not depend on scheduling.
+__global__ void __launch_bounds__(256) k_bias_grad(const float* __restrict__ dz, int B, int n, float* db) {
+    __shared__ float part[4][64];
+    const int cl = threadIdx.x & 63, k = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + cl;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    if (c < n) {
+        int b = k;
+        for (; b + 12 < B; b += 16) {
+            s0 += dz[(size_t)b * n + c];
+            s1 += dz[(size_t)(b + 4) * n + c];
+            s2 += dz[(size_t)(b + 8) * n + c];
+            s3 += dz[(size_t)(b + 12) * n + c];
+        }
+        for (; b < B; b += 4) s0 += dz[(size_t)b * n + c];
+    }
+    part[k][cl] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (k == 0 && c < n) db[c] = ((part[0][cl] + part[1][cl]) + part[2][cl]) + part[3][cl];
 }
 
 // Adam with bias correction; t = *counter + 1
@@ -200,6 +215,31 @@ __global__ void k_refresh(const float* __restrict__ W, int din, int dinp, int do
     const __bf16 v = bf(r < din ? W[(size_t)r * dout + c] : 0.f);
     Wb[(size_t)r * dout + c] = v;
     Wbt[(size_t)c * dinp + r] = v;
+}
+
+// all layers' copies in one launch (training replays this once per iteration, where a
+// launch per layer costs more than the copy): element i belongs to the layer l with
+// start[l] <= i < start[l + 1]
+constexpr int kRefreshLayers = 8;
+struct RefreshAll {
+    int L;
+    int64_t start[kRefreshLayers + 1];
+    const float* W[kRefreshLayers];
+    int din[kRefreshLayers], dinp[kRefreshLayers], dout[kRefreshLayers];
+    __bf16* Wb[kRefreshLayers];
+    __bf16* Wbt[kRefreshLayers];
+};
+
+__global__ void k_refresh_all(RefreshAll a) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= a.start[a.L]) return;
+    int l = 0;
+    while (g >= a.start[l + 1]) l++;
+    const int i = (int)(g - a.start[l]), dout = a.dout[l], dinp = a.dinp[l];
+    const int r = i / dout, c = i - r * dout;
+    const __bf16 v = bf(r < a.din[l] ? a.W[l][(size_t)r * dout + c] : 0.f);
+    a.Wb[l][(size_t)r * dout + c] = v;
+    a.Wbt[l][(size_t)c * dinp + r] = v;
 }
 
 __global__ void k_advance(int* counter) {
@@ -278,6 +318,21 @@ int ensure(mplb_mlp* n, T** ptr, size_t* cap, size_t count) {
 dim3 gemm_grid(int M, int N) { return dim3((unsigned)((N + kTile - 1) / kTile), (unsigned)((M + kTile - 1) / kTile)); }
 
 void refresh(mplb_mlp* n) {
+    if (n->L <= kRefreshLayers) {
+        RefreshAll a{};
+        a.L = n->L;
+        for (int l = 0; l < n->L; l++) {
+            a.start[l + 1] = a.start[l] + (int64_t)n->wp[l] * n->w[l + 1];
+            a.W[l] = n->p + n->woff[l];
+            a.din[l] = n->w[l];
+            a.dinp[l] = n->wp[l];
+            a.dout[l] = n->w[l + 1];
+            a.Wb[l] = n->Wb[l];
+            a.Wbt[l] = n->Wbt[l];
+        }
+        k_refresh_all<<<(unsigned)((a.start[n->L] + 255) / 256), 256, 0, n->stream>>>(a);
+        return;
+    }
     for (int l = 0; l < n->L; l++) {
         const int total = n->wp[l] * n->w[l + 1];
         k_refresh<<<(total + 255) / 256, 256, 0, n->stream>>>(n->p + n->woff[l], n->w[l], n->wp[l], n->w[l + 1],
@@ -332,7 +387,7 @@ void backward(mplb_mlp* n) {
         gw.ld32 = dout;
         gw.mvalid = n->w[l];
         k_gemm<E_DW><<<gemm_grid(gw.M, gw.N), 256, 0, n->stream>>>(gw);
-        k_bias_grad<<<(dout + 255) / 256, 256, 0, n->stream>>>(n->dz32[l + 1], B, dout, n->g + n->boff[l]);
+        k_bias_grad<<<(dout + 63) / 64, 256, 0, n->stream>>>(n->dz32[l + 1], B, dout, n->g + n->boff[l]);
         if (l > 0) {
             Gemm gx{};
             gx.A = n->dzb[l + 1];
