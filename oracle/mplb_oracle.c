@@ -875,7 +875,7 @@ static void litempmrt_collision(orc_case* c, int field) {
     double* fv = field_data(c, field);
     #pragma omp parallel for schedule(static)
     for (int64_t id = 0; id < c->N; id++) {
-        double n[15], n_eq[15], Gam[15];
+        double n[15] = {0}, n_eq[15], Gam[15];
         const double T = c->T[id];
         const double U[3] = {c->U[id * 3], c->U[id * 3 + 1], c->U[id * 3 + 2]};
         n_eq[0] = T;

@@ -327,3 +327,31 @@ def test_pressure_closed_forms():
     expect = (rho / 3.0 - 0.5 * psi ** 2 + 2.0 * sigma * b ** 2).reshape(12, 12)
     got = o.get("p").reshape(12, 12)
     assert np.allclose(got[1:-1, 1:-1], expect[1:-1, 1:-1], rtol=1e-9, atol=0)
+
+
+def test_pressure_one_sided_laplacian_at_walls():
+    """liMRTPressure.c:17-75: where a node lacks its x (or y) neighbour on one side, the
+    Laplacian takes the second difference one node inwards, psi(c) - 2 psi(b) + psi(x).
+    For psi quadratic in x and y every such difference is the exact second derivative, so
+    with sigma = 0 (no gradient term) every node of a walled box, edges and corners
+    included, has p = rho / 3 - psi^2 / 2 - psi (2 cx + 2 cy) / 12 (G = -1).  A one-sided
+    stencil that reused the node itself or pointed the wrong way would not give 2 c."""
+    from scipy.optimize import brentq
+    spec = CaseSpec(lbmodel=0, nx=12, ny=12, btype=0, f_model=0, f_Lambda=LAM9, f_sigma=0.0, g_model=None,
+                    write_interval=1)
+    o = O.Oracle(oracle_params(spec, ht=False))
+    tc = vdw_critical(0.5, 4.0)[1]
+    T = 0.8 * tc
+    o.set("T", np.full(o.N, T))
+    o.set("U", np.zeros((o.N, 3)))
+    psi_of = lambda r: np.sqrt(2.0 * (p_eos(r, T, a=0.5, b=4.0) - r / 3.0) / -1.0)   # G = -1
+    y, x = (v.ravel().astype(float) for v in np.indices((12, 12)))
+    cx, cy = 2e-4, 1.5e-4
+    psi = psi_of(0.012) + 1e-3 * x + cx * x ** 2 + 5e-4 * y + cy * y ** 2
+    rho = np.array([brentq(lambda r: psi_of(r) - s, 1e-4, 0.06, xtol=1e-16, rtol=1e-15) for s in psi])
+    o.set("rho", rho)
+    o.call("macro_pressure")
+    expect = rho / 3.0 - 0.5 * psi ** 2 - psi * (2.0 * cx + 2.0 * cy) / 12.0
+    np.testing.assert_allclose(o.get("p"), expect, rtol=1e-9, atol=0)
+    # the Laplacian term is resolved: dropping it moves the wall nodes far outside the bound
+    assert np.max(np.abs(o.get("p") - (rho / 3.0 - 0.5 * psi ** 2)) / expect) > 1e-6
