@@ -302,9 +302,11 @@ def test_generic_box_lists_that_leave_the_lattice_are_refused():
 def test_pressure_closed_forms():
     """liMRTPressure: p = rho cs2 + G psi^2 / 2 + G psi lap(psi) / 12 + 2 G^2 sigma |grad psi|^2.
     (i) A uniform field gives the equation of state, p = p_eos(rho, T), on every node.
-    (ii) A density chosen so that psi = psi0 + b x is linear gives, on interior nodes,
-    lap = 0 and grad = (b, 0): p = rho / 3 - psi^2 / 2 + 2 sigma b^2 (G = -1).  psi is
-    inverted from the van der Waals closed form, independently of the oracle."""
+    (ii) A density chosen so that psi = psi0 + bx x + by y is linear gives lap = 0 and
+    grad = (bx, by) on every node of the walled box, the one-sided first and second
+    differences at edges and corners included (both are exact for a linear psi):
+    p = rho / 3 - psi^2 / 2 + 2 sigma (bx^2 + by^2) (G = -1).  psi is inverted from the
+    van der Waals closed form, independently of the oracle."""
     from scipy.optimize import brentq
     sigma = 0.125
     spec = CaseSpec(lbmodel=0, nx=12, ny=12, btype=0, f_model=0, f_Lambda=LAM9, f_sigma=sigma, g_model=None,
@@ -318,15 +320,14 @@ def test_pressure_closed_forms():
     o.call("macro_pressure")
     assert np.allclose(o.get("p"), p_eos(0.1, T, a=0.5, b=4.0), rtol=1e-13, atol=0)
     psi_of = lambda r: np.sqrt(2.0 * (p_eos(r, T, a=0.5, b=4.0) - r / 3.0) / -1.0)   # G = -1
-    x = np.indices((12, 12))[1].ravel().astype(float)
-    psi0, b = psi_of(0.012), 4e-3
-    rho = np.array([brentq(lambda r: psi_of(r) - (psi0 + b * xi), 1e-4, 0.06, xtol=1e-16, rtol=1e-15) for xi in x])
-    o.set("rho", rho)
-    o.call("macro_pressure")
-    psi = psi0 + b * x
-    expect = (rho / 3.0 - 0.5 * psi ** 2 + 2.0 * sigma * b ** 2).reshape(12, 12)
-    got = o.get("p").reshape(12, 12)
-    assert np.allclose(got[1:-1, 1:-1], expect[1:-1, 1:-1], rtol=1e-9, atol=0)
+    y, x = (v.ravel().astype(float) for v in np.indices((12, 12)))
+    for bx, by in ((4e-3, 0.0), (2e-3, 1.5e-3)):
+        psi = psi_of(0.012) + bx * x + by * y
+        rho = np.array([brentq(lambda r: psi_of(r) - s, 1e-4, 0.06, xtol=1e-16, rtol=1e-15) for s in psi])
+        o.set("rho", rho)
+        o.call("macro_pressure")
+        expect = rho / 3.0 - 0.5 * psi ** 2 + 2.0 * sigma * (bx ** 2 + by ** 2)
+        np.testing.assert_allclose(o.get("p"), expect, rtol=1e-9, atol=0)
 
 
 def test_pressure_one_sided_laplacian_at_walls():
