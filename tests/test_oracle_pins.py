@@ -356,3 +356,44 @@ def test_pressure_one_sided_laplacian_at_walls():
     np.testing.assert_allclose(o.get("p"), expect, rtol=1e-9, atol=0)
     # the Laplacian term is resolved: dropping it moves the wall nodes far outside the bound
     assert np.max(np.abs(o.get("p") - (rho / 3.0 - 0.5 * psi ** 2)) / expect) > 1e-6
+
+
+@pytest.mark.parametrize("force", [0.0, 2e-4])
+def test_outflow_is_the_implicit_upwind_convective_condition(force):
+    """outflow.c is the convective condition df/dt + U df/dn = 0 taken implicit-upwind on
+    the outlet face: each population entering from outside (pulled from x - c with
+    c_x = -1 at x = nx - 1) becomes (a + U b) / (1 + U), a the node's own value, b the
+    inner neighbour's and U that neighbour's outward velocity (j + F / 2) / rho; the other
+    populations keep their values.  The neighbours hold the textbook D2Q9 equilibrium of a
+    known velocity, so U is known in closed form."""
+    n = 10
+    spec = CaseSpec(lbmodel=0, nx=n, ny=n, btype=0, f_model=0, f_Lambda=LAM9, g_model=None,
+                    f_bc={"X0": BC("bounceBack"), "X1": BC("outflow"), "Y0": BC("bounceBack"),
+                          "Y1": BC("bounceBack")})
+    o = O.Oracle(oracle_params(spec, ht=False))
+    rho0, u = 0.13, 0.04
+    o.set("rho", np.full(o.N, rho0))
+    o.set("U", np.zeros((o.N, 3)))
+    o.set("T", np.full(o.N, 0.03))
+    o.init()
+    t = O.lattice_tables(O.D2Q9)
+    c, w = t["vel"][:, :2], t["omega"]
+    cu = c[:, 0] * u
+    feq = w * rho0 * (1.0 + 3.0 * cu + 4.5 * cu ** 2 - 1.5 * u * u)
+    f = np.tile(feq, (n, n, 1))
+    rng = np.random.default_rng(11)
+    f[:, n - 1, :] = feq * (1.0 + 0.3 * rng.random((n, 9)))      # outlet nodes: arbitrary values
+    o.set("f", f.reshape(-1, 9))
+    Fi = np.zeros((n, n, 3))
+    Fi[:, n - 2, 0] = force                                        # on the inner neighbours
+    o.set("Fi", Fi.reshape(-1, 3))
+    before = o.get("f").reshape(n, n, 9).copy()
+    o.call("update_boundaries", 0)
+    after = o.get("f").reshape(n, n, 9)
+    U = u + 0.5 * force / rho0
+    entering = c[:, 0] == -1
+    for j in range(1, n - 1):                                      # outlet nodes off the corners
+        a, b = before[j, n - 1], before[j, n - 2]
+        np.testing.assert_allclose(after[j, n - 1, entering], ((a + U * b) / (1.0 + U))[entering], rtol=1e-13, atol=0)
+        np.testing.assert_array_equal(after[j, n - 1, ~entering], a[~entering])
+    np.testing.assert_array_equal(after[1:-1, 1:-2], before[1:-1, 1:-2])
