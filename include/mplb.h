@@ -127,7 +127,7 @@ int mplb_lattice_tables(int lbmodel, int* vel, int* rev, double* omega, double* 
  * (fixedRandomT.c:37,81): writes n successive values after srand(seed). */
 int mplb_glibc_rand(unsigned seed, int n, int32_t* out);
 
-/* Files in the reference's formats.  A case directory holding processor0/lattice/*
+/* Files in the reference's formats.  A case directory holding the processor0/lattice files
  * (written by latticeBox2D/3D + latticeMeshPartition for one rank) is opened from those
  * tables, as the solver's readLatticeMesh.c does; otherwise the box is built from
  * properties/latticeProperties. */

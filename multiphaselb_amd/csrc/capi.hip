@@ -417,13 +417,14 @@ struct TimedRegion {
     hipEvent_t a = nullptr, b = nullptr;
     TimedRegion(mplb_case* cc, int i) : c(cc), idx(i) {
         if (!c->timing) return;
-        hipEventCreate(&a);
-        hipEventCreate(&b);
-        hipEventRecord(a, c->stream);
+        // timing only: a failed event leaves this region out of the per-kernel times
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, c->stream);
     }
     ~TimedRegion() {
         if (!c->timing) return;
-        hipEventRecord(b, c->stream);
+        (void)hipEventRecord(b, c->stream);
         c->ev_pool.push_back(a);
         c->ev_pool.push_back(b);
         c->ev_pool.push_back((hipEvent_t)(intptr_t)idx);
@@ -437,15 +438,16 @@ void flush_timers(mplb_case* c) {
     if (c->timers.empty()) {
         for (int i = 0; i < MPLB_NTIMERS; i++) c->timers.push_back(Timer{kTimerNames[i], 0, 0});
     }
-    hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
     for (size_t i = 0; i + 2 < c->ev_pool.size(); i += 3) {
         float ms = 0;
-        hipEventElapsedTime(&ms, c->ev_pool[i], c->ev_pool[i + 1]);
         const int idx = (int)(intptr_t)c->ev_pool[i + 2];
-        c->timers[idx].ms += ms;
-        c->timers[idx].launches++;
-        hipEventDestroy(c->ev_pool[i]);
-        hipEventDestroy(c->ev_pool[i + 1]);
+        if (hipEventElapsedTime(&ms, c->ev_pool[i], c->ev_pool[i + 1]) == hipSuccess) {
+            c->timers[idx].ms += ms;
+            c->timers[idx].launches++;
+        }
+        (void)hipEventDestroy(c->ev_pool[i]);
+        (void)hipEventDestroy(c->ev_pool[i + 1]);
     }
     c->ev_pool.clear();
 }
@@ -788,18 +790,19 @@ int mplb_case_open(const char* case_dir, const mplb_opts* opts, mplb_case** out)
 
 int mplb_case_close(mplb_case* c) {
     if (!c) return MPLB_E_ARG;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    // teardown: failures here have nobody left to report to
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->fused) fused_destroy(c);
     void* ptrs[] = {c->f[0], c->f[1], c->f_alt[0], c->f_alt[1], c->rho, c->temp, c->p, c->U, c->Fi, c->Ft, c->ss, c->psi,
                     c->nb, c->onbnd, c->rnd, c->red, c->nanc, c->rho_snap};
     for (void* p : ptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (auto& b : c->bd) {
-        if (b.nodes) hipFree(b.nodes);
+        if (b.nodes) (void)hipFree(b.nodes);
         for (int f = 0; f < 2; f++)
-            if (b.rnd_off[f]) hipFree(b.rnd_off[f]);
+            if (b.rnd_off[f]) (void)hipFree(b.rnd_off[f]);
     }
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
 }
@@ -926,15 +929,16 @@ int mplb_time_steps(mplb_case* c, int nsteps, double* total_ms, double* kernel_m
     HIPCHK(c, hipEventRecord(a, c->stream));
     int done = 0;
     int rc = DISPATCH_T(c, steps_t<T>(c, nsteps, &done));
-    hipEventRecord(b, c->stream);
-    hipEventSynchronize(b);
+    hipError_t he = hipEventRecord(b, c->stream);
+    if (he == hipSuccess) he = hipEventSynchronize(b);
     float ms = 0;
-    hipEventElapsedTime(&ms, a, b);
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    if (he == hipSuccess) he = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     if (per_kernel) flush_timers(c);
     c->timing = false;
     if (rc) return rc;
+    if (he != hipSuccess) return fail(c, MPLB_E_HIP, std::string("timing events: ") + hipGetErrorString(he));
     if (total_ms) *total_ms = ms;
     for (int i = 0; i < MPLB_NTIMERS; i++) {
         if (kernel_ms) kernel_ms[i] = c->timers[i].launches ? c->timers[i].ms / c->timers[i].launches : 0.0;

@@ -1004,13 +1004,14 @@ int mplb_batch_launch_info(const mplb_batch* b, int* block, int* xcd_order, int*
 
 int mplb_batch_close(mplb_batch* b) {
     if (!b) return MPLB_E_ARG;
-    if (b->stream) hipStreamSynchronize(b->stream);
-    if (b->gexec) hipGraphExecDestroy(b->gexec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    // teardown: failures here have nobody left to report to
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     void* ptrs[] = {b->fa, b->fb, b->psi, b->rho, b->p, b->d_cases, b->d_work, b->d_mwork, b->d_delta};
     for (void* q : ptrs)
-        if (q) hipFree(q);
-    if (b->stream) hipStreamDestroy(b->stream);
+        if (q) (void)hipFree(q);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return 0;
 }

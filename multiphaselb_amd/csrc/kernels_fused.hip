@@ -888,13 +888,14 @@ struct Ev {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     Ev(mplb_case* cc, int i) : c(cc), idx(i) {
         if (!c->timing) return;
-        hipEventCreate(&e0);
-        hipEventCreate(&e1);
-        hipEventRecord(e0, c->stream);
+        // timing only: a failed event leaves that kernel class untimed
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e0, c->stream);
     }
     ~Ev() {
         if (!c->timing) return;
-        hipEventRecord(e1, c->stream);
+        (void)hipEventRecord(e1, c->stream);
         c->ev_pool.push_back(e0);
         c->ev_pool.push_back(e1);
         c->ev_pool.push_back((hipEvent_t)(intptr_t)idx);
@@ -957,8 +958,14 @@ int run_prologue(mplb_case* c) {
     const int Q = c->mesh.Q;
     const size_t bytes = sizeof(float) * Q * N;
     // collide copies of f and g (the generic arrays keep the current level)
-    float* tmp = nullptr;
-    FCHK(c, hipMalloc(&tmp, bytes));
+    struct Scratch {
+        float* p = nullptr;
+        ~Scratch() {
+            if (p) (void)hipFree(p);
+        }
+    } scratch;
+    FCHK(c, hipMalloc(&scratch.p, bytes));
+    float* tmp = scratch.p;
     FArgs a = make_args(c);
     for (int fld = 0; fld < (z->thermal ? 2 : 1); fld++) {
         FCHK(c, hipMemcpyAsync(tmp, c->f[fld], bytes, hipMemcpyDeviceToDevice, c->stream));
@@ -971,7 +978,6 @@ int run_prologue(mplb_case* c) {
         FCHK(c, hipMemcpyAsync(z->U[z->cur], c->U, sizeof(float) * 3 * N, hipMemcpyDeviceToDevice, c->stream));
     }
     FCHK(c, hipStreamSynchronize(c->stream));
-    hipFree(tmp);
     z->need_prologue = false;
     z->generic_current = true;
     return 0;
@@ -1250,24 +1256,24 @@ void fused_destroy(mplb_case* c) {
     void* ptrs[] = {z->fa, z->fb, z->ga, z->gb, z->rho, z->sumg, z->T[0], z->T[1], z->U[0], z->U[1],
                     z->bcf, z->bcg, z->bcTw, z->bcPert, z->rnd_off};
     for (void* p : ptrs)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);     // teardown: nothing left to report to
     delete z;
     c->fz = nullptr;
 }
 
-static void set_shifts(mplb_case* c) {
+static int set_shifts(mplb_case* c) {
     Fused* z = F(c);
-    if (!z->half) return;
+    if (!z->half) return 0;
     // deviation storage: f_k - w_k rho_ref, g_k - geq_k(T_ref, 0); reference values are
     // the lattice means of the initial fields
     const int64_t N = c->mesh.N;
     std::vector<float> h(N);
-    hipMemcpy(h.data(), c->rho, sizeof(float) * N, hipMemcpyDeviceToHost);
+    FCHK(c, hipMemcpy(h.data(), c->rho, sizeof(float) * N, hipMemcpyDeviceToHost));
     double rs = 0;
     for (float v : h) rs += v;
     const double rho_ref = rs / (double)N;
     double ts = 0;
-    hipMemcpy(h.data(), c->temp, sizeof(float) * N, hipMemcpyDeviceToHost);
+    FCHK(c, hipMemcpy(h.data(), c->temp, sizeof(float) * N, hipMemcpyDeviceToHost));
     for (float v : h) ts += v;
     const double T_ref = ts / (double)N;
     const int Q = c->mesh.Q;
@@ -1287,10 +1293,12 @@ static void set_shifts(mplb_case* c) {
         z->fshift[q] = (float)(w * rho_ref);
         z->gshift[q] = (float)geq[q];
     }
+    return 0;
 }
 
 int fused_after_init(mplb_case* c) {
-    set_shifts(c);
+    const int rc = set_shifts(c);
+    if (rc) return rc;
     F(c)->need_prologue = true;
     F(c)->generic_current = true;
     return dispatch(c, [&](auto L, auto S) { return run_prologue<decltype(L), decltype(S)>(c); });

@@ -755,7 +755,10 @@ void launch_collide(hipStream_t s, const EosParams& e, const ModelParams& mp, De
     const unsigned g = grid_for(st.N);
     switch (mp.model) {
     case M_LIMRT: k_collide_limrt<L, T><<<g, kBlock, 0, s>>>(st, e, mp, f); break;
-    case M_XUMRT: k_collide_xumrt<L, T><<<g, kBlock, 0, s>>>(st, e, mp, f, T(rho0)); break;
+    case M_XUMRT:
+        // D3Q15 only: case set-up refuses xuMRT on D2Q9 (the reference's own error)
+        if constexpr (L::Q == 15) k_collide_xumrt<L, T><<<g, kBlock, 0, s>>>(st, e, mp, f, T(rho0));
+        break;
     case M_LITEMPMRT: k_collide_litemp<L, T><<<g, kBlock, 0, s>>>(st, mp, f); break;
     case M_MYMRT: k_collide_my<L, T><<<g, kBlock, 0, s>>>(st, mp, f); break;
     default: break;
