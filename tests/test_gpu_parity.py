@@ -186,3 +186,55 @@ def test_config_shapes_1000_steps_match_oracle(name):
         r = max_rel(a, b)
         print("  %-3s max_rel %.3e exact %s" % (f, r, bool(np.array_equal(a, b))), flush=True)
         assert np.isfinite(a).all() and r <= TOL, (f, r)
+
+
+
+def _config0(a, b):
+    from multiphaselb_amd.initfields import maxwell, vdw_critical
+    T = 0.85 * vdw_critical(a, b)[1]
+    rv, rl = maxwell(T, a=a, b=b)
+    spec = d2(nx=128, ny=128, btype=3, a=a, b=b, g_model=None, write_interval=5000)
+    return spec, sphere_fields(spec, T=T, r_in=rl, r_out=rv, radius=30.0, seed=20240601), rv, rl
+
+
+def test_config0_droplet_20000_steps_match_oracle():
+    """The D2Q9 static droplet configuration at its own size and length: 128^2 fully
+    periodic, van der Waals at T = 0.85 Tc, radius 30 with a tanh profile of width 5
+    between the Maxwell densities, seeded +-1e-4 noise, zero velocity, G = -1, s_nu = 1,
+    liMRT (Li force scheme), 20000 FP64 steps with write steps every 5000 (pressure, rho_0
+    re-average): every field within 1e-12 of the oracle.  a = 0.5, b = 4 (the reference
+    examples' constants): with a = 9/49, b = 2/21 the potential is undefined at the
+    vapour density (next test)."""
+    spec, fields, rv, rl = _config0(0.5, 4.0)
+    c, o = pair(spec, fields, precision=M.FP64, ht=False)
+    print("case c0 128^2, Maxwell densities %.6f / %.6f" % (rv, rl), flush=True)
+    assert c.step(20000) == 20000 and o.run(20000) == 20000
+    assert o.error()[0] == 0, o.error()
+    for f in ("rho", "U", "f", "p"):
+        x, y = c.get(f), o.get(f)
+        r = max_rel(x, y)
+        print("  %-3s max_rel %.3e exact %s" % (f, r, bool(np.array_equal(x, y))), flush=True)
+        assert np.isfinite(x).all() and r <= TOL, (f, r)
+    # the droplet is still there: liquid inside, vapour outside
+    rho = c.get("rho").reshape(128, 128)
+    assert rho[64, 64] > 0.9 * rl and rho[2, 2] < 1.2 * rv
+
+
+def test_config0_literal_constants_are_undefined_as_in_the_reference():
+    """With a = 9/49, b = 2/21, R = 1 at T = 0.85 Tc the Maxwell vapour density (1.119)
+    lies where p - rho cs2 > 0, so psi = sqrt(2 (p - rho cs2) / G) (potential.c) is NaN
+    there: after one step the library and the oracle hold NaN at exactly the same nodes
+    and identical values elsewhere, and the write step refuses the field with the
+    reference's message."""
+    spec, fields, rv, rl = _config0(9.0 / 49.0, 2.0 / 21.0)
+    c, o = pair(spec, fields, precision=M.FP64, ht=False)
+    assert c.step(1) == 1 and o.run(1) == 1
+    x, y = c.get("rho"), o.get("rho")
+    nan = np.isnan(y)
+    print("NaN nodes after one step: %d of %d" % (int(nan.sum()), nan.size), flush=True)
+    assert nan.any() and not nan.all()
+    np.testing.assert_array_equal(np.isnan(x), nan)
+    np.testing.assert_array_equal(x[~nan], y[~nan])
+    with pytest.raises(M.MplbError) as e:
+        c.step(5000)
+    assert "NaN" in str(e.value)
