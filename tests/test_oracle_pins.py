@@ -397,3 +397,46 @@ def test_outflow_is_the_implicit_upwind_convective_condition(force):
         np.testing.assert_allclose(after[j, n - 1, entering], ((a + U * b) / (1.0 + U))[entering], rtol=1e-13, atol=0)
         np.testing.assert_array_equal(after[j, n - 1, ~entering], a[~entering])
     np.testing.assert_array_equal(after[1:-1, 1:-2], before[1:-1, 1:-2])
+
+
+@pytest.mark.parametrize("lbmodel,lam3,a1,a2", [(0, 1.2, 0.25, 0.25), (0, 1.0, -1.0, 1.0),
+                                                 (1, 1.2, 0.25, 0.25), (1, 1.0, -0.5, 0.5)])
+def test_mymrt_temperature_diffuses_at_the_heat_source_conductivity(lbmodel, lam3, a1, a2):
+    """myMRT (myMRTCollision.c:35-212, with its off-diagonal relaxation entries) is a
+    diffusion scheme: with the flow frozen, uniform density and zero velocity the heat
+    source vanishes and a sine temperature profile of wavenumber k decays as
+    exp(-kappa k^2 t), where kappa = (1/Lambda3 - 1/2)(4 + 3 a1 + 2 a2)/6 (D2Q9) or
+    (1/Lambda3 - 1/2)(6 + 11 a1 + a2)/9 (D3Q15), the conductivity the reference's
+    heatSource.c assumes.  The measured rate agrees to lattice discretisation error
+    (< 4e-3 here); dropping the off-diagonal entry gives 0.81 of it, flipping its sign
+    0.62."""
+    if lbmodel == 0:
+        n, reps = 128, 4
+        gl = (0.0, 1.0, 1.0, lam3, 1.0, lam3, 1.0, 1.0, 1.0)
+        spec = CaseSpec(lbmodel=0, nx=n, ny=4, btype=3, f_model=0, g_model=1, g_Lambda=gl, g_alpha1=a1,
+                        g_alpha2=a2, write_interval=10 ** 6)
+        kappa = (1.0 / lam3 - 0.5) * (4.0 + 3.0 * a1 + 2.0 * a2) / 6.0
+    else:
+        n, reps = 96, 16
+        gl = (0.0, 1.0, 1.0, lam3, 1.0, lam3, 1.0, lam3, 1.0) + (1.0,) * 6
+        lam15 = (1.0, 1.1, 1.1, 1.0, 1.1, 1.0, 1.1, 1.0, 1.1, 1.0, 1.0, 1.0, 1.0, 1.0, 1.2)
+        spec = CaseSpec(lbmodel=1, nx=n, ny=4, nz=4, btype=5, f_model=2, f_Lambda=lam15, g_model=1, g_Lambda=gl,
+                        g_alpha1=a1, g_alpha2=a2, write_interval=10 ** 6)
+        kappa = (1.0 / lam3 - 0.5) * (6.0 + 11.0 * a1 + a2) / 9.0
+    o = O.Oracle(oracle_params(spec, ht=True, frozen=True))
+    P = n - 1                                   # duplicate-node period
+    x = np.tile(np.arange(n), reps).astype(float)
+    T0, eps = 0.03, 1e-4
+    o.set("rho", np.full(o.N, 0.1))
+    o.set("U", np.zeros((o.N, 3)))
+    o.set("T", T0 + eps * np.sin(2.0 * np.pi * x / P))
+    o.init()
+    ts, amp = [], []
+    for k in range(10):
+        assert o.run(200) == 200
+        T = o.get("T")[:P]
+        ts.append(200.0 * (k + 1))
+        amp.append(2.0 / P * np.sum((T - T0) * np.sin(2.0 * np.pi * np.arange(P) / P)))
+    rate = -np.polyfit(ts[1:], np.log(amp[1:]), 1)[0]
+    assert np.all(o.get("U") == 0.0)
+    assert abs(rate / (kappa * (2.0 * np.pi / P) ** 2) - 1.0) < 4e-3
