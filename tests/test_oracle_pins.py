@@ -331,7 +331,7 @@ def test_pressure_closed_forms():
 
 
 def test_pressure_one_sided_laplacian_at_walls():
-    """liMRTPressure.c:17-75: where a node lacks its x (or y) neighbour on one side, the
+    """liMRTPressure.c:16-134: where a node lacks its x (or y) neighbour on one side, the
     Laplacian takes the second difference one node inwards, psi(c) - 2 psi(b) + psi(x).
     For psi quadratic in x and y every such difference is the exact second derivative, so
     with sigma = 0 (no gradient term) every node of a walled box, edges and corners
@@ -440,3 +440,42 @@ def test_mymrt_temperature_diffuses_at_the_heat_source_conductivity(lbmodel, lam
     rate = -np.polyfit(ts[1:], np.log(amp[1:]), 1)[0]
     assert np.all(o.get("U") == 0.0)
     assert abs(rate / (kappa * (2.0 * np.pi / P) ** 2) - 1.0) < 4e-3
+
+
+@pytest.mark.parametrize("lam3,lam4,u", [(1.2, 1.0, 0.05), (1.0, 1.5, 0.05), (1.2, 1.0, 0.0)])
+def test_litempmrt_advects_temperature_at_the_corrected_speed(lam3, lam4, u):
+    """liTempMRTCollision.c:75-77 adds (1 - Lambda3/2) Lambda4 n4 to n3 using the whole
+    moment n4 (the form with the non-equilibrium part n4 - n4_eq is commented out at :71-73).
+    For a uniform field the moment fixed point is then n4 = -T u, n3 = T u (1 - (1 -
+    Lambda3/2) Lambda4 / Lambda3): in a frozen uniform flow a sine temperature profile
+    travels at that fraction of u (0.667 u for Lambda = (1.2, 1.0), 0.25 u for (1.0, 1.5);
+    the commented-out form would give u) and decays at the D2Q9 advection-diffusion rate
+    cs2 (1/Lambda3 - 1/2) k^2 (cs2 = 1/3, from e_eq = -2T)."""
+    n = 128
+    gl = (0.0, 1.0, 1.0, lam3, lam4, lam3, lam4, 1.0, 1.0)
+    spec = CaseSpec(lbmodel=0, nx=n, ny=4, btype=3, f_model=0, g_model=3, g_Lambda=gl, write_interval=10 ** 6)
+    o = O.Oracle(oracle_params(spec, ht=True, frozen=True))
+    P = n - 1
+    k = 2.0 * np.pi / P
+    x = np.tile(np.arange(n), 4).astype(float)
+    U = np.zeros((o.N, 3))
+    U[:, 0] = u
+    T0 = 0.03
+    o.set("rho", np.full(o.N, 0.1))
+    o.set("U", U)
+    o.set("T", T0 + 1e-4 * np.sin(k * x))
+    o.init()
+    xs = np.arange(P)
+    ts, amp, phase = [], [], []
+    for it in range(10):
+        assert o.run(200) == 200
+        T = o.get("T")[:P] - T0
+        a, b = np.sum(T * np.sin(k * xs)), np.sum(T * np.cos(k * xs))
+        ts.append(200.0 * (it + 1))
+        amp.append(2.0 / P * np.hypot(a, b))
+        phase.append(np.arctan2(b, a))               # T ~ sin(k (x - c t)): phase = -k c t
+    rate = -np.polyfit(ts[1:], np.log(amp[1:]), 1)[0]
+    np.testing.assert_allclose(rate, (1.0 / lam3 - 0.5) / 3.0 * k * k, rtol=1e-2)
+    speed = -np.polyfit(ts[1:], np.unwrap(phase[1:]), 1)[0] / k
+    expect = u * (1.0 - (1.0 - lam3 / 2.0) * lam4 / lam3)
+    assert abs(speed - expect) <= 5e-3 * u + 1e-9, (speed, expect)
