@@ -479,3 +479,48 @@ def test_litempmrt_advects_temperature_at_the_corrected_speed(lam3, lam4, u):
     speed = -np.polyfit(ts[1:], np.unwrap(phase[1:]), 1)[0] / k
     expect = u * (1.0 - (1.0 - lam3 / 2.0) * lam4 / lam3)
     assert abs(speed - expect) <= 5e-3 * u + 1e-9, (speed, expect)
+
+
+def test_fixed_u_is_the_non_equilibrium_extrapolation_with_a_forced_neighbour():
+    """fixedU.c (liMRT) is the non-equilibrium extrapolation of Guo, Zheng and Shi (2002):
+    each population k missing at a wall node x becomes f_eq_k(rho_n, U_w) + (f_k(n) -
+    f_eq_k(rho_n, u_n)), n = x + c_k the inner node it would have come from reversed,
+    rho_n and u_n = (sum c f(n) + F(n) / 2) / rho_n from that node's current populations
+    and force.  Here the inner nodes hold arbitrary non-equilibrium populations and a
+    force, and the expectation uses the textbook D2Q9 equilibrium."""
+    n = 10
+    Uw = np.array([0.02, -0.005, 0.0])
+    spec = CaseSpec(lbmodel=0, nx=n, ny=n, btype=0, f_model=0, f_Lambda=LAM9, g_model=None,
+                    f_bc={"X0": BC("fixedU", tuple(Uw)), "X1": BC("bounceBack"), "Y0": BC("bounceBack"),
+                          "Y1": BC("bounceBack")})
+    o = O.Oracle(oracle_params(spec, ht=False))
+    o.set("rho", np.full(o.N, 0.13))
+    o.set("U", np.zeros((o.N, 3)))
+    o.set("T", np.full(o.N, 0.03))
+    o.init()
+    t = O.lattice_tables(O.D2Q9)
+    c, w = t["vel"][:, :2].astype(float), t["omega"]
+
+    def feq(rho, u):
+        cu = c @ u[:2]
+        return w * rho * (1.0 + 3.0 * cu + 4.5 * cu ** 2 - 1.5 * np.dot(u[:2], u[:2]))
+
+    rng = np.random.default_rng(5)
+    f = o.get("f").reshape(n, n, 9) * (1.0 + 0.2 * rng.random((n, n, 9)))
+    Fi = np.zeros((n, n, 3))
+    Fi[:, :, :2] = 1e-3 * rng.standard_normal((n, n, 2))
+    o.set("f", f.reshape(-1, 9))
+    o.set("Fi", Fi.reshape(-1, 3))
+    before = o.get("f").reshape(n, n, 9).copy()
+    o.call("update_boundaries", 0)
+    after = o.get("f").reshape(n, n, 9)
+    for j in range(2, n - 2):                   # X0 nodes whose inner neighbours are interior
+        for k in np.nonzero(c[:, 0] == 1)[0]:     # pulled from x - c: outside at x = 0
+            jn, inn = j + int(c[k, 1]), 1
+            fn = before[jn, inn]
+            rho_n = fn.sum()
+            u_n = (c.T @ fn + 0.5 * Fi[jn, inn, :2]) / rho_n
+            want = feq(rho_n, Uw)[k] + fn[k] - feq(rho_n, u_n)[k]
+            np.testing.assert_allclose(after[j, 0, k], want, rtol=1e-13, atol=0)
+        others = c[:, 0] != 1
+        np.testing.assert_array_equal(after[j, 0, others], before[j, 0, others])
