@@ -22,10 +22,18 @@
  *                       expansion F ~ -G psi cs2-weighted grad psi.
  *   boundaries          bounceBack mass conservation in closed boxes and the
  *                       tangential-average fixed point; fixedT/fixedU reproduce the
- *                       wall value when the neighbour is at equilibrium; glibc rand()
- *                       stream for fixedRandomT taken from the host libc.
+ *                       wall value when the neighbour is at equilibrium; fixedU is the
+ *                       non-equilibrium extrapolation (Guo, Zheng, Shi 2002) next to
+ *                       forced non-equilibrium neighbours; outflow is the implicit-upwind
+ *                       convective condition on the outlet face; glibc rand() stream for
+ *                       fixedRandomT taken from the host libc.
  *   heatSource          vanishes for uniform T, U; divergence/gradient stencils exact on
  *                       linear fields.
+ *   myMRT / liTempMRT   frozen-flow transport: a sine temperature decays at the closed-form
+ *                       diffusivity (myMRT: the heat source's conductivity) and liTempMRT
+ *                       advects at the fixed-point speed of its whole-moment correction.
+ *   macroPressure       closed forms on uniform, linear and quadratic potentials, wall and
+ *                       corner nodes included.
  * Functions or quirks that none of these fixes are marked "parity unpinned" where they
  * occur below.
  */
@@ -1095,7 +1103,9 @@ static void bc_fixedU(orc_case* c, double* fv, int bid, int model, const double 
     }
 }
 
-/* boundary/outflow/outflow.c:12-208 (normal accumulated in float, as the reference) */
+/* boundary/outflow/outflow.c:12-208 (normal accumulated in float, as the reference).
+ * Parity unpinned: at the outlet's corner nodes the float normal truncates to the rest
+ * direction, so the node is its own neighbour (no closed form fixes that choice). */
 static void bc_outflow(orc_case* c, double* fv, int bid) {
     const orc_lattice* L = &c->lat;
     const int Q = L->Q;
@@ -1317,6 +1327,8 @@ void orc_macro_pressure(orc_case* c) {
                 }
             }
             pr = pr + G * psi * lap / 12;
+            /* parity unpinned: the sign of the one-sided gradient at walls (the pressure
+             * takes only |grad psi|^2, which the linear-potential pin fixes) */
             double grad[3] = {0, 0, 0};
             if (c->lat.model == ORC_D2Q9) {
                 for (int ax = 0; ax < 2; ax++) {
