@@ -1327,8 +1327,8 @@ void orc_macro_pressure(orc_case* c) {
                 }
             }
             pr = pr + G * psi * lap / 12;
-            /* parity unpinned: the sign of the one-sided gradient at walls (the pressure
-             * takes only |grad psi|^2, which the linear-potential pin fixes) */
+            /* the sign of the one-sided gradient at walls is unpinned but reaches no
+             * output: only |grad psi|^2 is used, which the linear-potential pin fixes */
             double grad[3] = {0, 0, 0};
             if (c->lat.model == ORC_D2Q9) {
                 for (int ax = 0; ax < 2; ax++) {
