@@ -1103,9 +1103,9 @@ static void bc_fixedU(orc_case* c, double* fv, int bid, int model, const double 
     }
 }
 
-/* boundary/outflow/outflow.c:12-208 (normal accumulated in float, as the reference).
- * Parity unpinned: at the outlet's corner nodes the float normal truncates to the rest
- * direction, so the node is its own neighbour (no closed form fixes that choice). */
+/* boundary/outflow/outflow.c:12-208 (normal accumulated in float, as the reference: at
+ * an outlet corner the normal's components truncate to 0, the node is its own neighbour
+ * and keeps its populations) */
 static void bc_outflow(orc_case* c, double* fv, int bid) {
     const orc_lattice* L = &c->lat;
     const int Q = L->Q;

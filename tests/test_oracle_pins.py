@@ -524,3 +524,33 @@ def test_fixed_u_is_the_non_equilibrium_extrapolation_with_a_forced_neighbour():
             np.testing.assert_allclose(after[j, 0, k], want, rtol=1e-13, atol=0)
         others = c[:, 0] != 1
         np.testing.assert_array_equal(after[j, 0, others], before[j, 0, others])
+
+
+def test_outflow_leaves_outlet_corners_unchanged():
+    """At an outlet corner outflow.c's normal is the float sum of two axis directions,
+    (1, 1)/sqrt(2) (or (1, -1)/sqrt(2)); converting its components to int truncates them
+    to 0, so the matching lattice direction is the rest one and the node is its own
+    neighbour: (a + U a) / (1 + U) = a leaves every population as it was, up to the
+    rounding of that expression.  (A normal rounded to the diagonal would mix in the
+    diagonal neighbour, a change of order 10% here.)  The Y patches carry no condition
+    here, so only outflow touches the corners."""
+    n = 10
+    spec = CaseSpec(lbmodel=0, nx=n, ny=n, btype=0, f_model=0, f_Lambda=LAM9, g_model=None,
+                    f_bc={"X0": BC("bounceBack"), "X1": BC("outflow"), "Y0": BC("periodic"),
+                          "Y1": BC("periodic")})
+    o = O.Oracle(oracle_params(spec, ht=False))
+    o.set("rho", np.full(o.N, 0.13))
+    U = np.zeros((o.N, 3))
+    U[:, 0] = 0.03
+    o.set("U", U)
+    o.set("T", np.full(o.N, 0.03))
+    o.init()
+    rng = np.random.default_rng(9)
+    f = o.get("f").reshape(n, n, 9) * (1.0 + 0.3 * rng.random((n, n, 9)))
+    o.set("f", f.reshape(-1, 9))
+    before = o.get("f").reshape(n, n, 9).copy()
+    o.call("update_boundaries", 0)
+    after = o.get("f").reshape(n, n, 9)
+    for j in (0, n - 1):
+        np.testing.assert_allclose(after[j, n - 1], before[j, n - 1], rtol=1e-15, atol=0)
+    assert not np.array_equal(after[1:-1, n - 1], before[1:-1, n - 1])    # the face itself is updated
