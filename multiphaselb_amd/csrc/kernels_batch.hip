@@ -311,6 +311,11 @@ __device__ inline int wrap_dup(int u, int n) { return u < 0 ? u + (n - 1) : (u >
 // psi never goes to memory.  Per node and step the populations are read once and written
 // once.  LDS hazards: row y + 1 goes to slot (y + 2 - y0) & 3, which no lane reads while
 // colliding row y - 1 or y, and one barrier per row publishes it.
+// wave priority while a row's populations are pulled (as in the 3D single pass: loads
+// go out ahead of other waves' collisions; 0.9% faster in FP64, neutral in FP32)
+#ifndef MPLB_BATCH_PRIO
+#define MPLB_BATCH_PRIO 2
+#endif
 template <typename S, typename T>
 __global__ void __launch_bounds__(kMarchLanes) k_batch_march(BArgs a, const MItem* items, int per, int nitems) {
     __shared__ T ring[4][kMarchLanes];
@@ -331,6 +336,7 @@ __global__ void __launch_bounds__(kMarchLanes) k_batch_march(BArgs a, const MIte
     // populations and psi of raw row yr (y0 - 1 <= yr <= y1)
     auto row = [&](int yr) {
         if (!act) return;
+        if (MPLB_BATCH_PRIO) __builtin_amdgcn_s_setprio(MPLB_BATCH_PRIO);
         const int yw = wrap_dup(yr, ny);
         const unsigned Y[3] = {(unsigned)((yw == 0 ? ny - 2 : yw - 1) * nx), (unsigned)(yw * nx),
                                (unsigned)((yw == ny - 1 ? 1 : yw + 1) * nx)};
@@ -340,6 +346,7 @@ __global__ void __launch_bounds__(kMarchLanes) k_batch_march(BArgs a, const MIte
             fn[q] = ld<S, T>(a.fin, q * a.P + bc.off + l, shift_of<T>(bc, q));
         }
         ring[(yr - it.y0 + 1) & 3][t] = potential<T>(a.eos, density<T>(fn), T(bc.T));
+        if (MPLB_BATCH_PRIO) __builtin_amdgcn_s_setprio(0);
     };
     row(it.y0 - 1);
     row(it.y0);
