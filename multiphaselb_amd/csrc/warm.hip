@@ -74,7 +74,7 @@ __global__ void k_radial(Radial a, T* __restrict__ rho, double* __restrict__ par
     s[threadIdx.x] = v;
     __syncthreads();
     for (int h = kBlock / 2; h > 0; h >>= 1) {
-        if (threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+        if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
@@ -95,7 +95,7 @@ __global__ void k_change(const T* __restrict__ x, T* __restrict__ snap, int64_t 
     s[threadIdx.x] = u;
     __syncthreads();
     for (int h = kBlock / 2; h > 0; h >>= 1) {
-        if (threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] > s[threadIdx.x + h] ? s[threadIdx.x] : s[threadIdx.x + h];
+        if ((int)threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] > s[threadIdx.x + h] ? s[threadIdx.x] : s[threadIdx.x + h];
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
