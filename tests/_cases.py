@@ -1,6 +1,5 @@
 """Shared case builders for the parity tests: one CaseSpec drives both the library
 (through a case directory in the reference's format) and the CPU oracle."""
-import os
 import tempfile
 
 import numpy as np

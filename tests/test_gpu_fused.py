@@ -6,7 +6,7 @@ import pytest
 import multiphaselb_amd as M
 from _cases import make_case_dir, pair, sphere_fields
 from multiphaselb_amd.initfields import maxwell, noise, sphere
-from test_gpu_parity import CASES, TC, d2, d3
+from test_gpu_parity import CASES, TC, d3
 
 pytestmark = pytest.mark.gpu
 
@@ -66,7 +66,9 @@ def test_fused_readback_does_not_perturb_the_run():
 def test_march_matches_two_pass(monkeypatch):
     """The single-pass kernel follows the two-pass kernels on a periodic D3Q15 bubble;
     psi is computed with the same arithmetic in both, so the trajectories agree to FP32
-    contraction differences."""
+    contraction differences: 1e-5 relative on rho and f, and 1e-6 absolute on U (lattice
+    units; a rounding-level difference moves U by ~1e-7, a wrong population by ~1e-3,
+    while max |U| is itself small, so a bound relative to it would be fragile)."""
     spec, fk, ht, frozen, steps = CASES["d3q15_bubble_nht"]
     fields = sphere_fields(spec, seed=11, **fk)
     monkeypatch.setenv("MPLB_FUSED_KERNEL", "twopass")
@@ -75,10 +77,13 @@ def test_march_matches_two_pass(monkeypatch):
     one, _ = pair(spec, fields, precision=M.FP32, ht=False, fused=True)
     two.step(steps)
     one.step(steps)
-    for f in ("rho", "f", "U"):
+    for f in ("rho", "f"):
         e = rel(one.get(f), two.get(f))
         print("  %s rel diff %.2e" % (f, e), flush=True)
         assert e < 1e-5
+    du = float(np.max(np.abs(one.get("U") - two.get("U"))))
+    print("  U abs diff %.2e (max |U| %.2e)" % (du, float(np.max(np.abs(two.get("U"))))), flush=True)
+    assert du < 1e-6
 
 
 @pytest.mark.parametrize("kernel", ["single-pass", "twopass"])
