@@ -713,7 +713,7 @@ def main():
     s_mac = 8 if args.precision == "fp64" else 4
     kname = "fused" if "fused" in r["kern"] else "collide"
     k_ms, k_launches = r["kern"].get(kname, (0.0, 0))
-    # isothermal periodic D3Q15 with FP32 storage runs the single-pass kernel (psi stays
+    # isothermal periodic D3Q15 (FP32 or FP16 storage) runs the single-pass kernel (psi stays
     # on chip): one timed kernel, where the two-pass kernels also report their first pass
     single = kname == "fused" and not args.thermal and "fused_aux" not in r["kern"]
     if kname == "collide":

@@ -1,7 +1,7 @@
 // Fused structured-box path: FP32 arithmetic, FP32 or FP16-deviation population storage.
 //
 // The state between steps is the post-collision population set of the current step.
-// Isothermal D3Q15 boxes with FP32 storage whose links are all the periodic wrap take
+// Isothermal D3Q15 boxes (FP32 or FP16 storage) whose links are all the periodic wrap take
 // one single-pass kernel per step (k_fused_march: pull-stream, psi in an LDS ring of
 // planes, force, velocity and collision).  Other boxes take two node-parallel passes (three with heat
 // transfer):
@@ -862,12 +862,10 @@ FArgs make_args(mplb_case* c) {
     return a;
 }
 
-// The single-pass kernel serves FP32 storage only: with FP16 storage the two-pass
-// kernels are faster (the step is no longer bound by the population bytes).
+// The single-pass kernel (D3Q15, FP32 or FP16 storage)
 template <class L, typename S, bool OUT>
 void launch_march(mplb_case* c, const FArgs& a) {
-    if constexpr (L::D == 3 && std::is_same_v<S, float>)
-        k_fused_march<L, S, OUT><<<dim3(8u * (unsigned)a.mper), 256, 0, c->stream>>>(a);
+    if constexpr (L::D == 3) k_fused_march<L, S, OUT><<<dim3(8u * (unsigned)a.mper), 256, 0, c->stream>>>(a);
 }
 
 template <class L, typename S>
@@ -1211,14 +1209,14 @@ int fused_create(mplb_case* c) {
         }
         z->face_table = mask;
     }
-    // single-pass kernel: FP32 storage, isothermal D3Q15 with every link the periodic
-    // wrap and no boundary node (MPLB_FUSED_KERNEL=twopass keeps the two-pass kernels)
+    // single-pass kernel: isothermal D3Q15 with every link the periodic wrap and no
+    // boundary node (MPLB_FUSED_KERNEL=twopass keeps the two-pass kernels)
     {
         const char* env = std::getenv("MPLB_FUSED_KERNEL");
         const bool twopass = env && std::strcmp(env, "twopass") == 0;
         bool any_bnd = false;
         for (int64_t id = 0; id < N && !any_bnd; id++) any_bnd = m.onbnd[id] != 0;
-        z->march = !twopass && !z->half && m.lbmodel == 1 && !z->thermal && z->face_table == 0 && !any_bnd && m.nx >= 3 &&
+        z->march = !twopass && m.lbmodel == 1 && !z->thermal && z->face_table == 0 && !any_bnd && m.nx >= 3 &&
                    m.ny >= 3 && m.nz >= 3;
         if (z->march) {
             z->mtx = (m.nx + MX - 1) / MX;
@@ -1227,7 +1225,10 @@ int fused_create(mplb_case* c) {
             // `resident` workgroups (CUs x workgroups per CU at the kernel's occupancy), so
             // pick the count that minimises rounds x (planes per chunk + 2 halo planes)
             int bpc = 0, cus = 0;
-            FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march<D3Q15, float, false>, 256, 0));
+            if (z->half)
+                FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march<D3Q15, _Float16, false>, 256, 0));
+            else
+                FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march<D3Q15, float, false>, 256, 0));
             FCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->opts.device));
             const long long resident = std::max(1LL, (long long)bpc * cus);
             int nchunks = 1;

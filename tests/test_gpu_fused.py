@@ -63,18 +63,20 @@ def test_fused_readback_does_not_perturb_the_run():
         assert np.array_equal(a.get(f), b.get(f)), f
 
 
-def test_march_matches_two_pass(monkeypatch):
-    """The single-pass kernel follows the two-pass kernels on a periodic D3Q15 bubble;
-    psi is computed with the same arithmetic in both, so the trajectories agree to FP32
+@pytest.mark.parametrize("precision", [M.FP32, M.FP16S])
+def test_march_matches_two_pass(precision, monkeypatch):
+    """The single-pass kernel follows the two-pass kernels on a periodic D3Q15 bubble, with
+    FP32 and FP16 storage; psi is computed with the same arithmetic in both, so the
+    trajectories agree to FP32
     contraction differences: 1e-5 relative on rho and f, and 1e-6 absolute on U (lattice
     units; a rounding-level difference moves U by ~1e-7, a wrong population by ~1e-3,
     while max |U| is itself small, so a bound relative to it would be fragile)."""
     spec, fk, ht, frozen, steps = CASES["d3q15_bubble_nht"]
     fields = sphere_fields(spec, seed=11, **fk)
     monkeypatch.setenv("MPLB_FUSED_KERNEL", "twopass")
-    two, _ = pair(spec, fields, precision=M.FP32, ht=False, fused=True)
+    two, _ = pair(spec, fields, precision=precision, ht=False, fused=True)
     monkeypatch.delenv("MPLB_FUSED_KERNEL")
-    one, _ = pair(spec, fields, precision=M.FP32, ht=False, fused=True)
+    one, _ = pair(spec, fields, precision=precision, ht=False, fused=True)
     two.step(steps)
     one.step(steps)
     for f in ("rho", "f"):
