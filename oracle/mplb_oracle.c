@@ -28,7 +28,8 @@
  *                       convective condition on the outlet face; glibc rand() stream for
  *                       fixedRandomT taken from the host libc.
  *   heatSource          vanishes for uniform T, U; divergence/gradient stencils exact on
- *                       linear fields.
+ *                       linear fields; with myMRT it makes the steady flux rho dT/dy
+ *                       uniform across a static density profile.
  *   myMRT / liTempMRT   frozen-flow transport: a sine temperature decays at the closed-form
  *                       diffusivity (myMRT: the heat source's conductivity) and liTempMRT
  *                       advects at the fixed-point speed of its whole-moment correction.
