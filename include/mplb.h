@@ -221,6 +221,16 @@ int mplb_density_change(mplb_case* c, double* max_abs);
  * first (period n - 1) and is left out of the sum. */
 int mplb_set_radial_density(mplb_case* c, const double* prof, int nprof, double d0, double dd, const double* centre,
                             double R, double* mass);
+/* mplb_set_radial_density without any device allocation (no hipMalloc/hipFree): the
+ * profile table and the partial masses live in a fixed device scratch array of the
+ * library (552 KiB per GPU, held until the process ends).  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, allocation-free serving or stepping loops).  Limits: nprof <= 4096 and
+ * at most 2^24 stored nodes (MPLB_E_ARG beyond them).  Results are identical to
+ * mplb_set_radial_density's.  All calls on one GPU share that GPU's scratch: calls that
+ * overlap on one GPU (several host threads) race. */
+int mplb_set_radial_density_noalloc(mplb_case* c, const double* prof, int nprof, double d0, double dd,
+                                    const double* centre, double R, double* mass);
 
 /* Multilayer perceptron (tanh hidden layers, linear output, mean squared error, Adam)
  * whose products run on the matrix cores with BF16 operands and FP32 accumulation;

@@ -59,7 +59,7 @@ EXPORTS = [
     "mplb_batch_init", "mplb_batch_step", "mplb_batch_run_to_steady", "mplb_batch_get_field",
     "mplb_batch_case_steps", "mplb_batch_tune", "mplb_batch_set_launch", "mplb_batch_launch_info",
     "mplb_batch_set_march", "mplb_batch_march_rows",
-    "mplb_density_change", "mplb_set_radial_density",
+    "mplb_density_change", "mplb_set_radial_density", "mplb_set_radial_density_noalloc",
     "mplb_write_lattice", "mplb_write_fields", "mplb_read_field",
     "mplb_mlp_create", "mplb_mlp_close", "mplb_mlp_last_error", "mplb_mlp_num_params", "mplb_mlp_set_params",
     "mplb_mlp_get_params", "mplb_mlp_forward", "mplb_mlp_gradient", "mplb_mlp_train", "mplb_mlp_tensor",
@@ -121,6 +121,7 @@ def library():
         "mplb_write_fields": ([vp, cp], i),
         "mplb_read_field": ([vp, cp, i, cp, i, vp, C.c_size_t], i),
         "mplb_set_radial_density": ([vp, vp, i, d, d, vp, d, C.POINTER(d)], i),
+        "mplb_set_radial_density_noalloc": ([vp, vp, i, d, d, vp, d, C.POINTER(d)], i),
         "mplb_mlp_create": ([i, vp, i, i, C.POINTER(vp)], i),
         "mplb_mlp_close": ([vp], i),
         "mplb_mlp_last_error": ([vp], cp),
@@ -259,13 +260,14 @@ class Case:
         self._check(self.L.mplb_density_change(self.h, C.byref(v)))
         return v.value
 
-    def set_radial_density(self, prof, d0, dd, centre, R):
-        """Writes rho = prof(|x - centre| - R) into the case; returns the total mass."""
+    def set_radial_density(self, prof, d0, dd, centre, R, noalloc=False):
+        """Writes rho = prof(|x - centre| - R) into the case; returns the total mass.
+        noalloc: use the entry point that allocates no device memory."""
         pr = np.ascontiguousarray(prof, dtype=np.float64)
         cen = np.ascontiguousarray(list(centre) + [0.0] * (3 - len(centre)), dtype=np.float64)
         m = C.c_double()
-        self._check(self.L.mplb_set_radial_density(self.h, pr.ctypes.data, pr.size, d0, dd, cen.ctypes.data, R,
-                                                    C.byref(m)))
+        fn = self.L.mplb_set_radial_density_noalloc if noalloc else self.L.mplb_set_radial_density
+        self._check(fn(self.h, pr.ctypes.data, pr.size, d0, dd, cen.ctypes.data, R, C.byref(m)))
         return m.value
 
     def average_density(self):

@@ -321,6 +321,10 @@ int setup_device(mplb_case* c) {
     const int Q = m.Q;
     HIPCHK(c, hipSetDevice(c->opts.device));
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {
+        const int rc = warm_scratch_setup(c);
+        if (rc) return rc;
+    }
     c->scalar_bytes = c->opts.precision == MPLB_FP64 ? 8 : 4;
     const size_t sb = c->scalar_bytes;
     for (int i = 0; i < 2; i++) {

@@ -87,3 +87,9 @@ struct mplb_case {
     std::vector<mplb::Timer> timers;
     std::vector<hipEvent_t> ev_pool;
 };
+
+namespace mplb {
+// resolve the device address of warm.hip's allocation-free scratch array for the
+// case's device (once per device; the device must be current)
+int warm_scratch_setup(mplb_case* c);
+}  // namespace mplb

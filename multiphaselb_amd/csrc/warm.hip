@@ -8,10 +8,17 @@
 //                            the mass of the cold initial field;
 //   mplb_density_change      max_x |rho(x) - rho_prev(x)| of the current time level
 //                            against the previous call's snapshot (device reduction).
+//   mplb_set_radial_density_noalloc
+//                            the radial raster without any device allocation: the
+//                            profile table, the per-block partial masses and the total
+//                            live in this file's `scratch` array (one copy per GPU,
+//                            held until the process ends), so callers that may not
+//                            allocate (graph capture, allocation-free loops) can use it.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/mplb.h"
@@ -23,6 +30,21 @@ using namespace mplb;
 namespace {
 
 constexpr int kBlock = 256;
+
+// Scratch of the allocation-free raster: [0, kMaxProf) profile table, [kMaxProf,
+// kMaxProf + kMaxPart) one partial mass per block, then the ordered total.  Limits:
+// nprof <= kMaxProf and N <= kMaxPart * kBlock (2^24 nodes, a 256^3 box).  552 KiB per
+// GPU.  Every call on a GPU shares that GPU's copy and leaves its contents behind;
+// calls that overlap on one GPU (several host threads or streams) race.
+constexpr int kMaxProf = 4096;
+constexpr int64_t kMaxPart = 65536;
+constexpr int64_t kScratchDoubles = kMaxProf + kMaxPart + 32;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// device address of `scratch` per device ordinal (the host-side name is only a stand-in)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
 
 int wfail(mplb_case* c, int code, const std::string& m) {
     if (c) c->err = m;
@@ -80,6 +102,29 @@ __global__ void k_radial(Radial a, T* __restrict__ rho, double* __restrict__ par
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
+// total of the per-block partials in block order, as the host loop of
+// mplb_set_radial_density adds them (same FP64 additions, same result)
+__global__ void k_sum_in_order(const double* __restrict__ part, unsigned n, double* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0;
+    for (unsigned b = 0; b < n; b++) s += part[b];
+    *out = s;
+}
+
+int radial_dup(const mplb_case* c) {
+    // periodic axes of the built boxes (latticeBox2D/3D boundaryType); none for lattices
+    // read from files
+    switch (c->mesh.btype) {
+    case BOX_PERIODIC_X: return 1;
+    case BOX_PERIODIC_Y: return 2;
+    case BOX_PERIODIC_XY: return 3;
+    case BOX_PERIODIC_Z: return 4;
+    case BOX_PERIODIC_XYZ: return 7;
+    case BOX_PERIODIC_XY_SPOT: return 3;
+    default: return 0;
+    }
+}
+
 // per-block max of |x - snap| as the bit pattern of a non-negative double (a NaN
 // difference orders above every number), then snap := x
 template <typename T>
@@ -103,6 +148,22 @@ __global__ void k_change(const T* __restrict__ x, T* __restrict__ snap, int64_t 
 
 }  // namespace
 
+namespace mplb {
+
+int warm_scratch_setup(mplb_case* c) {
+    const int dev = c->opts.device;
+    if (dev < 0 || dev >= kMaxDevices) return wfail(c, MPLB_E_ARG, "device ordinal out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[dev]) return 0;
+    void* p = nullptr;
+    WCHK(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return wfail(c, MPLB_E_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_dev[dev] = (double*)p;
+    return 0;
+}
+
+}  // namespace mplb
+
 extern "C" {
 
 int mplb_set_radial_density(mplb_case* c, const double* prof, int nprof, double d0, double dd, const double* centre,
@@ -124,20 +185,8 @@ int mplb_set_radial_density(mplb_case* c, const double* prof, int nprof, double 
     WCHK(c, hipMalloc(&dprof, sizeof(double) * nprof));
     WCHK(c, hipMalloc(&part, sizeof(double) * nb));
     WCHK(c, hipMemcpyAsync(dprof, prof, sizeof(double) * nprof, hipMemcpyHostToDevice, c->stream));
-    // periodic axes of the built boxes (latticeBox2D/3D boundaryType); none for lattices
-    // read from files
-    int dup = 0;
-    switch (c->mesh.btype) {
-    case BOX_PERIODIC_X: dup = 1; break;
-    case BOX_PERIODIC_Y: dup = 2; break;
-    case BOX_PERIODIC_XY: dup = 3; break;
-    case BOX_PERIODIC_Z: dup = 4; break;
-    case BOX_PERIODIC_XYZ: dup = 7; break;
-    case BOX_PERIODIC_XY_SPOT: dup = 3; break;
-    default: dup = 0;
-    }
     Radial a{N, c->mesh.nx, c->mesh.ny, c->mesh.nz, centre[0], centre[1], c->mesh.d == 3 ? centre[2] : 0.0, R, d0, dd,
-             dprof, nprof, dup};
+             dprof, nprof, radial_dup(c)};
     if (c->scalar_bytes == 8) k_radial<double><<<nb, kBlock, 0, c->stream>>>(a, (double*)c->rho, part);
     else k_radial<float><<<nb, kBlock, 0, c->stream>>>(a, (float*)c->rho, part);
     WCHK(c, hipGetLastError());
@@ -149,6 +198,49 @@ int mplb_set_radial_density(mplb_case* c, const double* prof, int nprof, double 
     WCHK(c, hipFree(part));
     double s = 0;
     for (unsigned b = 0; b < nb; b++) s += hp[b];
+    if (mass) *mass = s;
+    c->initialized = false;       // the new density takes effect at mplb_init
+    return 0;
+}
+
+int mplb_set_radial_density_noalloc(mplb_case* c, const double* prof, int nprof, double d0, double dd,
+                                    const double* centre, double R, double* mass) {
+    if (!c || !prof || nprof < 2 || !(dd > 0) || !centre) return MPLB_E_ARG;
+    if (c->opts.device < 0 || !c->rho) return wfail(c, MPLB_E_STATE, "the case has no device fields (device < 0)");
+    if (!c->mesh.structured) return wfail(c, MPLB_E_MODEL, "radial fields need a lattice in structured box order");
+    if (nprof > kMaxProf)
+        return wfail(c, MPLB_E_ARG, "allocation-free raster: nprof " + std::to_string(nprof) + " exceeds the scratch limit " +
+                                        std::to_string(kMaxProf));
+    const int64_t N = c->mesh.N;
+    if (N > kMaxPart * kBlock)
+        return wfail(c, MPLB_E_ARG, "allocation-free raster: " + std::to_string(N) + " nodes exceed the scratch limit " +
+                                        std::to_string(kMaxPart * kBlock));
+    double* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (c->opts.device < kMaxDevices) base = g_scratch_dev[c->opts.device];
+    }
+    if (!base) return wfail(c, MPLB_E_STATE, "scratch address not resolved for this device");
+    WCHK(c, hipSetDevice(c->opts.device));
+    if (c->fused) {
+        const int rc = fused_set_field(c, "rho", nullptr, 0);   // launches only, no allocation
+        if (rc < 0) return rc;
+    }
+    const unsigned nb = (unsigned)((N + kBlock - 1) / kBlock);
+    double* dprof = base;
+    double* part = base + kMaxProf;
+    double* total = base + kMaxProf + kMaxPart;
+    WCHK(c, hipMemcpyAsync(dprof, prof, sizeof(double) * nprof, hipMemcpyHostToDevice, c->stream));
+    Radial a{N, c->mesh.nx, c->mesh.ny, c->mesh.nz, centre[0], centre[1], c->mesh.d == 3 ? centre[2] : 0.0, R, d0, dd,
+             dprof, nprof, radial_dup(c)};
+    if (c->scalar_bytes == 8) k_radial<double><<<nb, kBlock, 0, c->stream>>>(a, (double*)c->rho, part);
+    else k_radial<float><<<nb, kBlock, 0, c->stream>>>(a, (float*)c->rho, part);
+    WCHK(c, hipGetLastError());
+    k_sum_in_order<<<1, 64, 0, c->stream>>>(part, nb, total);
+    WCHK(c, hipGetLastError());
+    double s = 0;
+    WCHK(c, hipMemcpyAsync(&s, total, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    WCHK(c, hipStreamSynchronize(c->stream));
     if (mass) *mass = s;
     c->initialized = false;       // the new density takes effect at mplb_init
     return 0;
