@@ -250,7 +250,8 @@ int mplb_mlp_forward(mplb_mlp* n, const float* x, int rows, float* y);
 /* loss and gradient of one batch (x [batch][in], t [batch][out]); grad may be NULL */
 int mplb_mlp_gradient(mplb_mlp* n, const float* x, const float* t, float* loss, float* grad, size_t ngrad);
 /* iters Adam steps on minibatches idx[it * batch .. +batch) of the samples X/T, replayed
- * from one captured graph; loss_hist[it] (may be NULL) is each minibatch's loss. */
+ * from one captured graph; loss_hist[it] (may be NULL) is each minibatch's loss.  Each
+ * call starts a fresh Adam run: moments zeroed, bias-correction step count from 1. */
 int mplb_mlp_train(mplb_mlp* n, const float* X, const float* T, int nsamples, const int32_t* idx, int iters,
                    float lr, float beta1, float beta2, float eps, float* loss_hist);
 /* intermediate tensors of the last gradient call as FP32: "a" (level j BF16 input/

@@ -44,12 +44,12 @@ struct FArgs {
     const int* nb;                              // [Q][N]
     const uint8_t* onbnd;
     const uint8_t* bcf;                         // per-node boundary code, f (surface nodes)
-    const uint8_t* bcg;
+    const uint8_t* bcg;                         // low 4 bits: code; fixedRandomT: list slot in the high 4
     const float* bcTw;                          // per-node wall temperature (g)
     const float* bcPert;                        // per-node fixedRandomT percentage
     const int* rnd_off;                         // per-node first draw
     const double* rnd;
-    int draw;                                   // fixedRandomT drawing this step
+    int draw;                                   // fixedRandomT lists drawing this step (bit per list slot)
     EosParams eos;
     float rho0;
     int fmodel, gmodel, surf;
@@ -214,7 +214,7 @@ __device__ inline void bounce_back(const FArgs& a, const NodePos& p, float* v) {
 // fixedT.c:45-122 / fixedRandomT.c:48-112 in registers; the neighbour's post-stream
 // population is pulled directly and its T and U are the previous step's.
 template <class L, typename S>
-__device__ inline void fixed_temperature(const FArgs& a, const NodePos& p, float* v, uint8_t code) {
+__device__ inline void fixed_temperature(const FArgs& a, const NodePos& p, float* v, uint8_t code, bool draws) {
     int draw = code == FB_FRDT ? a.rnd_off[p.id] : 0;
     const float Tw0 = a.bcTw[p.id];
     float minT = 0, maxT = 0;
@@ -236,7 +236,7 @@ __device__ inline void fixed_temperature(const FArgs& a, const NodePos& p, float
         else litemp_geq<L, float>(Tnb, U, geq);
         const float eq_nb = geq[q];
         float Tw = Tw0;
-        if (code == FB_FRDT && a.draw) Tw = minT + (maxT - minT) * (float)a.rnd[draw++];
+        if (code == FB_FRDT && draws) Tw = minT + (maxT - minT) * (float)a.rnd[draw++];
         if (a.gmodel == 1) my_geq<L, float>(Tw, U, a.a1, a.a2, geq);
         else litemp_geq<L, float>(Tw, U, geq);
         v[q] = geq[q] + (gnb - eq_nb);
@@ -253,9 +253,10 @@ template <class L, typename S>
 __device__ inline void pull_g(const FArgs& a, const NodePos& p, float* v) {
     pull<L, S>(a, p, a.gin, a.gshift, v);
     if (!p.table) return;
-    const uint8_t code = a.bcg[p.id];
+    const uint8_t bc = a.bcg[p.id];
+    const uint8_t code = bc & 0xF;
     if (code == FB_BBACK) bounce_back<L>(a, p, v);
-    else if (code == FB_FT || code == FB_FRDT) fixed_temperature<L, S>(a, p, v, code);
+    else if (code == FB_FT || code == FB_FRDT) fixed_temperature<L, S>(a, p, v, code, ((a.draw >> (bc >> 4)) & 1) != 0);
 }
 
 // K1
@@ -762,7 +763,8 @@ struct Fused {
     std::vector<std::pair<int, int>> frdt_lists;   // (field, list) of fixedRandomT
     bool need_prologue = true;     // post-collision buffers must be rebuilt from the generic state
     bool generic_current = true;   // generic device arrays describe the current time level
-    int last_draw = 0;             // fixedRandomT drawing flag of the last step
+    int last_draw = 0;             // fixedRandomT drawing lists of the last step (bit per slot)
+    float rho0_built = 0.f;        // rho_0 the post-collision buffers were computed with
     int64_t P = 0;                 // population plane pitch
     int face_table = 0;            // faces whose links differ from the periodic wrap
     float fshift[15] = {0}, gshift[15] = {0};
@@ -988,6 +990,7 @@ int run_prologue(mplb_case* c) {
     FCHK(c, hipStreamSynchronize(c->stream));
     z->need_prologue = false;
     z->generic_current = true;
+    z->rho0_built = (float)c->rho0;
     return 0;
 }
 
@@ -1004,14 +1007,16 @@ int do_step(mplb_case* c) {
     a.gin = z->ga;
     a.gout = z->gb;
     a.first_call = c->current == 0 ? 1 : 0;
-    // fixedRandomT: draws while localTStep < psteps (one flag for all such lists)
+    // fixedRandomT (fixedRandomT.c:79): each list draws while its own localTStep < psteps
     a.draw = 0;
-    for (auto& fl : z->frdt_lists) {
-        FieldCfg& fc = c->fc[fl.first];
-        if (fc.localT[fl.second] < fc.psteps[fl.second]) a.draw = 1;
-        fc.localT[fl.second]++;
+    for (size_t s = 0; s < z->frdt_lists.size(); s++) {
+        FieldCfg& fc = c->fc[z->frdt_lists[s].first];
+        const int b = z->frdt_lists[s].second;
+        if (fc.localT[b] < fc.psteps[b]) a.draw |= 1 << s;
+        fc.localT[b]++;
     }
     z->last_draw = a.draw;
+    z->rho0_built = a.rho0;
     if (c->timing) launch_step_timed<L, S>(c, a, z->thermal, false);
     else launch_step<L, S>(c, a, z->thermal, false);
     std::swap(z->fa, z->fb);
@@ -1117,6 +1122,17 @@ bool fused_supported(const mplb_case* c, std::string* why) {
         }
     }
     if (thermal) {
+        // g list of every node; fixedRandomT lists get a 4-bit slot in the kernels
+        std::vector<int> glist(m.N, -1);
+        int nfrdt = 0;
+        for (size_t b = 0; b < m.bd.size(); b++) {
+            const int t = c->fc[1].bctype[b];
+            if (t == BD_FRDT) nfrdt++;
+            if (t == BD_PERIODIC) continue;
+            for (int32_t id : m.bd[b]) glist[id] = (int)b;
+        }
+        if (nfrdt > 16) return no("more than 16 fixedRandomT lists");
+        const int naxis = m.lbmodel == 0 ? 4 : 6;
         for (size_t b = 0; b < m.bd.size(); b++) {
             const int t = c->fc[1].bctype[b];
             if (t != BD_FT && t != BD_FRDT) continue;
@@ -1124,8 +1140,15 @@ bool fused_supported(const mplb_case* c, std::string* why) {
                 for (int k = 0; k < m.Q; k++) {
                     if (m.nb[(int64_t)id * m.Q + k] != -1) continue;
                     const int32_t nbid = m.nb[(int64_t)id * m.Q + reverse_of(c, k)];
-                    if (nbid != -1 && (seen[nbid] & 2) && m.nb[(int64_t)nbid * m.Q + k] == -1)
+                    if (nbid == -1 || !(seen[nbid] & 2)) continue;
+                    if (m.nb[(int64_t)nbid * m.Q + k] == -1)
                         return no("fixedT reads a population another condition writes");
+                    // bounceBack.c:34-44 averages present axis pairs: a bounceBack list applied
+                    // before this one rewrites the neighbour's link k that fixedT reads
+                    const int nl = glist[nbid];
+                    if (nl >= 0 && nl < (int)b && c->fc[1].bctype[nl] == BD_BBACK && k >= 1 && k <= naxis &&
+                        m.nb[(int64_t)nbid * m.Q + reverse_of(c, k)] != -1)
+                        return no("fixedT reads a population an earlier bounceBack list averages");
                 }
         }
     }
@@ -1168,8 +1191,9 @@ int fused_create(mplb_case* c) {
             uint8_t code = t == BD_BBACK ? FB_BBACK : t == BD_FT ? FB_FT : t == BD_FRDT ? FB_FRDT : FB_NONE;
             if (code == FB_NONE) continue;
             int cnt = 0;
+            const uint8_t slot = code == FB_FRDT ? (uint8_t)(z->frdt_lists.size() << 4) : 0;
             for (int32_t id : m.bd[b]) {
-                (f == 0 ? bcf : bcg)[id] = code;
+                (f == 0 ? bcf : bcg)[id] = code | slot;
                 if (f == 1) {
                     tw[id] = (float)fc.bcval[b][0];
                     pert[id] = (float)fc.pert[b];
@@ -1325,9 +1349,14 @@ int fused_sync_to_generic(mplb_case* c) {
 }
 
 int fused_after_write(mplb_case* c) {
-    // rho_0 may have changed: the next step's collision is recomputed from the current
-    // level on the next fused step
-    F(c)->need_prologue = true;
+    // The write step may have re-averaged rho_0 (multiPhasePP.c:249-315).  rho_0 enters
+    // only the gravity term.  liMRT's next collision uses Ft from the step's interForce,
+    // i.e. the old rho_0, which is what the post-collision buffers already hold; xuMRT's
+    // collision forms Fi + (rho - rho_0) g with the new value (xuMRTForce.c:18-69), so
+    // only then is the next step's collision recomputed from the current level.
+    Fused* z = F(c);
+    const bool gravity = c->eos.g[0] != 0 || c->eos.g[1] != 0 || c->eos.g[2] != 0;
+    if (gravity && c->fc[0].mp.model == M_XUMRT && (float)c->rho0 != z->rho0_built) z->need_prologue = true;
     return 0;
 }
 

@@ -574,8 +574,8 @@ int mplb_mlp_train(mplb_mlp* n, const float* X, const float* T, int nsamples, co
     MCHK(n, hipMemcpyAsync(n->X, X, sizeof(float) * nsamples * din, hipMemcpyHostToDevice, n->stream));
     MCHK(n, hipMemcpyAsync(n->T, T, sizeof(float) * nsamples * dout, hipMemcpyHostToDevice, n->stream));
     MCHK(n, hipMemcpyAsync(n->idx, idx, sizeof(int32_t) * iters * B, hipMemcpyHostToDevice, n->stream));
-    // the iteration counter restarts for this call's index table and loss history; the
-    // Adam moments carry over, and the step count of the bias correction restarts too
+    // every call is a fresh Adam run: the iteration counter (index table, loss history,
+    // bias-correction step count) restarts and the first and second moments are zeroed
     hipLaunchKernelGGL(k_set_int, 1, 1, 0, n->stream, n->counter, 0);
     MCHK(n, hipMemsetAsync(n->m, 0, sizeof(float) * n->P, n->stream));
     MCHK(n, hipMemsetAsync(n->v, 0, sizeof(float) * n->P, n->stream));

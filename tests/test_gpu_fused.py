@@ -1,5 +1,7 @@
 """Fused structured-box kernels (FP32 and FP16-deviation storage) against the FP64
 oracle and against the per-routine FP32 kernels, through the C ABI."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -11,7 +13,8 @@ from test_gpu_parity import CASES, TC, d3
 pytestmark = pytest.mark.gpu
 
 FUSED_CASES = ["d2q9_droplet_nht", "d2q9_bubble_thermal", "d2q9_column_fixedT", "d2q9_column_liTempMRT",
-               "d2q9_surface_tension", "d3q15_bubble_nht", "d3q15_thermal_column", "d3q15_surface_tension"]
+               "d2q9_surface_tension", "d3q15_bubble_nht", "d3q15_thermal_column", "d3q15_surface_tension",
+               "d2q9_column_two_frdt", "d3q15_column_frdt_nopert"]
 
 
 def rel(a, b):
@@ -48,8 +51,8 @@ def test_fused_fp16_storage_error(name):
 
 
 def test_fused_readback_does_not_perturb_the_run():
-    """Reading fields mid-run (materialisation) and write steps leave the trajectory
-    bit-identical to an uninterrupted run."""
+    """Reading fields mid-run (materialisation) leaves the trajectory bit-identical to an
+    uninterrupted run (both runs take the same write steps)."""
     spec, fk, ht, frozen, steps = CASES["d3q15_thermal_column"]
     fields = sphere_fields(spec, seed=5, **fk)
     a, _ = pair(spec, fields, precision=M.FP32, ht=ht, fused=True)
@@ -114,3 +117,18 @@ def test_fp32_mass_does_not_drift(kernel, monkeypatch):
         m = float(np.asarray(c.get("rho"), np.float64)[sites].sum())
         print("  %s: after %d steps mass rel change %.2e" % (kernel, 1000 * (chunk + 1), m / m0 - 1.0), flush=True)
         assert abs(m / m0 - 1.0) < 1e-5
+
+
+@pytest.mark.parametrize("name", ["d3q15_bubble_nht", "d2q9_bubble_thermal"])
+def test_fused_write_interval_leaves_the_trajectory_unchanged(name):
+    """Without gravity rho_0 enters no collision, so write steps (pressure, rho_0
+    average, NaN check) change nothing later steps read: write intervals of 7 and of the
+    case's own give bit-identical FP32 trajectories."""
+    spec, fk, ht, frozen, steps = CASES[name]
+    fields = sphere_fields(spec, seed=5, **fk)
+    a, _ = pair(spec, fields, precision=M.FP32, ht=ht, fused=True)
+    b, _ = pair(dataclasses.replace(spec, write_interval=7), fields, precision=M.FP32, ht=ht, fused=True)
+    a.step(steps)
+    b.step(steps)
+    for f in ("rho", "U", "T", "f"):
+        assert np.array_equal(a.get(f), b.get(f)), f

@@ -66,6 +66,18 @@ CASES = {
                                 g_bc={"Z0": BC("fixedRandomT", (0.95 * TC,), pert=0.5, duration=50),
                                       "Z1": BC("fixedT", (0.8 * TC,))}),
                              dict(T=0.85 * TC, r_in=RL, r_out=RV, radius=0, column=5.0), True, False, 120),
+    # two fixedRandomT lists that stop drawing at different steps (fixedRandomT.c:79 is per list)
+    "d2q9_column_two_frdt": (d2(nx=5, ny=40, btype=1, g=(0.0, -1e-6, 0.0), rho0=1.0, write_interval=100,
+                                f_bc={"Y0": BC("bounceBack"), "Y1": BC("bounceBack")},
+                                g_bc={"Y0": BC("fixedRandomT", (0.95 * TC,), pert=0.5, duration=7),
+                                      "Y1": BC("fixedRandomT", (0.8 * TC,), pert=1.0, duration=40)}),
+                             dict(T=0.85 * TC, r_in=RL, r_out=RV, radius=0, column=14.0), True, False, 120),
+    # a fixedRandomT list without pert never draws, next to one that does
+    "d3q15_column_frdt_nopert": (d3(nx=6, ny=6, nz=16, btype=3, g=(0.0, 0.0, -1e-6), rho0=1.0, write_interval=40,
+                                    f_bc={"Z0": BC("bounceBack"), "Z1": BC("bounceBack")},
+                                    g_bc={"Z0": BC("fixedRandomT", (0.95 * TC,), pert=0.5, duration=50),
+                                          "Z1": BC("fixedRandomT", (0.8 * TC,))}),
+                                 dict(T=0.85 * TC, r_in=RL, r_out=RV, radius=0, column=5.0), True, False, 80),
     "d3q15_surface_tension": (d3(nx=10, ny=10, nz=10, btype=5, f_st=1, f_kappa=0.3, write_interval=50),
                               dict(T=0.85 * TC, r_in=RL, r_out=RV, radius=3.0), False, False, 60),
     "d3q15_closed_box": (d3(nx=8, ny=8, nz=8, btype=0, write_interval=30,
