@@ -235,6 +235,10 @@ std::string build_box(Mesh& m, double spot_radius, const double spot_centre[2]) 
         }
         case BOX_PERIODIC_Z: {                              // periodicZ3D.c:6-251
             if (nz != nx) return "periodicZ boundary lists are only defined for Nz == Nx";
+            // periodicZ3D.c:86-93 fills nz*(nx-2) of the (nx-2)*ny Y-list entries (the rest
+            // stay unset for ny > nz) and :221-236 completes node j*ny + (nz-1)*nx*ny, which
+            // leaves the lattice for ny >= nx + 2
+            if (ny > nz) return "periodicZ boundary lists are only defined for Ny <= Nz";
             auto& x0 = b.list("X0", (size_t)nxy);
             auto& x1 = b.list("X1", (size_t)nxy);
             auto& y0 = b.list("Y0", (size_t)(nx - 2) * ny);

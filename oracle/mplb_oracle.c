@@ -478,6 +478,9 @@ static void build_boundaries(orc_case* c) {
     }
     case ORC_PERIODIC_Z: {               /* periodicZ3D.c:6-251; well defined only for nz == nx */
         if (nz != nx) { orc_fail(c, "periodicZ boundary lists are only defined for Nz == Nx"); return; }
+        /* Y-list entries beyond nz*(nx-2) stay unset for ny > nz; node j*ny + (nz-1)*nx*ny
+         * leaves the lattice for ny >= nx + 2 */
+        if (ny > nz) { orc_fail(c, "periodicZ boundary lists are only defined for Ny <= Nz"); return; }
         c->nbd = 4;
         int *x0 = bd_alloc(c, 0, nx * ny), *x1 = bd_alloc(c, 1, nx * ny);
         int *y0 = bd_alloc(c, 2, (nx - 2) * ny), *y1 = bd_alloc(c, 3, (nx - 2) * ny);

@@ -128,6 +128,87 @@ def test_periodic_tables_are_duplicate_node_wrap(n):
         assert exact_miss == (4 if lbmodel == 0 else 0)
 
 
+def _ref_box():
+    path = os.path.join(O.ORACLE_DIR, "_ref", "librefbox.so")
+    if not os.path.exists(path):
+        if os.path.isdir("/root/reference"):
+            subprocess.check_call(["make", "-s", "-C", O.ORACLE_DIR, "ref"])
+        else:
+            pytest.skip("oracle/_ref not built and reference tree absent")
+    R = ctypes.CDLL(path)
+    R.refbox_build.argtypes = [ctypes.c_int] * 6 + [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_int64, ctypes.POINTER(ctypes.c_long)]
+    return R
+
+
+def _search_table(lbmodel, nx, ny, nz):
+    """The neighbour search of latticeBox2D.c:148-239 / latticeBox3D.c:166-215, restated
+    from its definition: the upstream node x - c_k when it lies inside the box, else -1."""
+    t = M.lattice_tables(lbmodel)
+    nzz = nz if lbmodel == 1 else 1
+    k, j, i = np.meshgrid(np.arange(nzz), np.arange(ny), np.arange(nx), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    Q = len(t["rev"])
+    nb = np.full((i.size, Q), -1, np.int32)
+    for q in range(Q):
+        c = t["vel"][q]
+        ui, uj, uk = i - c[0], j - c[1], k - c[2]
+        ok = (ui >= 0) & (ui < nx) & (uj >= 0) & (uj < ny) & (uk >= 0) & (uk < nzz)
+        nb[ok, q] = (ui + uj * nx + uk * nx * ny)[ok]
+    return nb
+
+
+REF_BOXES = [
+    (0, 7, 9, 1, 0), (0, 8, 8, 1, 0), (0, 9, 8, 1, 0), (0, 5, 12, 1, 0),
+    (0, 11, 6, 1, 1), (0, 3, 12, 1, 1), (0, 8, 8, 1, 1), (0, 6, 13, 1, 1),
+    (0, 7, 7, 1, 2), (0, 5, 9, 1, 2), (0, 6, 11, 1, 2),
+    (0, 10, 9, 1, 3), (0, 7, 7, 1, 3), (0, 4, 11, 1, 3), (0, 12, 5, 1, 3),
+    (1, 6, 5, 7, 0), (1, 5, 6, 8, 0), (1, 4, 4, 4, 0), (1, 7, 6, 5, 0),
+    (1, 5, 6, 8, 3), (1, 6, 5, 7, 3), (1, 7, 7, 4, 3),
+    (1, 6, 6, 6, 4), (1, 5, 5, 5, 4), (1, 6, 4, 6, 4), (1, 7, 5, 7, 4),
+    (1, 6, 5, 7, 5), (1, 9, 8, 5, 5), (1, 4, 4, 4, 5),
+]
+
+
+@pytest.mark.parametrize("lbmodel,nx,ny,nz,btype", REF_BOXES)
+def test_box_tables_match_the_reference_routines(lbmodel, nx, ny, nz, btype):
+    """The reference's own genericBoundary*/periodic*.c, compiled into oracle/_ref from
+    their sources and run on the neighbour search's table, give the same completed
+    neighbour table and boundary lists (contents and order) as the library and the
+    oracle, bit for bit.  periodicX.c also writes outside its lists (its loop runs j from
+    0 to ny-1 into entries j-1 of (ny-2)-long lists), and so does periodicZ3D.c for
+    ny < nz (nz*(nx-2) entries into (nx-2)*ny-long Y lists); the guarded allocator sees
+    both."""
+    R = _ref_box()
+    nb = _search_table(lbmodel, nx, ny, nz)
+    N, Q = nb.shape
+    sizes = np.zeros(6, np.int32)
+    lists = np.zeros(8 * N, np.int32)
+    oob = ctypes.c_long(0)
+    n = R.refbox_build(3 if lbmodel == 1 else 2, btype, nx, ny, nz, Q, nb.ctypes.data, 6, sizes.ctypes.data,
+                       lists.ctypes.data, lists.size, ctypes.byref(oob))
+    assert n >= 0
+    ref_lists = []
+    at = 0
+    for b in range(n):
+        ref_lists.append(lists[at:at + sizes[b]].copy())
+        at += sizes[b]
+    assert (oob.value > 0) == ((lbmodel == 0 and btype == 1) or (lbmodel == 1 and btype == 4 and ny < nz))
+    spec = CaseSpec(lbmodel=lbmodel, nx=nx, ny=ny, nz=nz, btype=btype, f_model=0 if lbmodel == 0 else 2,
+                    f_Lambda=(1.0,) * (9 if lbmodel == 0 else 15),
+                    g_Lambda=(0.0,) + (1.0,) * (8 if lbmodel == 0 else 14), g_model=1)
+    c = M.Case(make_case_dir(spec), device=-1)
+    o = O.Oracle(oracle_params(spec))
+    assert o.error()[0] == 0
+    assert np.array_equal(c.neighbours(), nb)
+    assert np.array_equal(o.nb(), nb)
+    lib_lists = [a for _, a in c.boundaries()]
+    assert len(lib_lists) == n and len(o.boundaries()) == n
+    for a, b, r in zip(lib_lists, o.boundaries(), ref_lists):
+        assert np.array_equal(a, r)
+        assert np.array_equal(b, r)
+
+
 def _parse(case):
     d = os.path.join(GOLD, "dicts", case)
     return M.Case(d, device=-1)
@@ -187,3 +268,15 @@ def test_generic_2d_y1_list_outside_lattice_is_rejected(tmp_path):
     spec = CaseSpec(lbmodel=0, nx=9, ny=7, btype=0)
     with pytest.raises(M.MplbError):
         M.Case(make_case_dir(spec, root=str(tmp_path)), device=-1)
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(5, 7, 5), (6, 7, 6)])
+def test_periodic_z_shapes_the_reference_leaves_undefined_are_refused(tmp_path, nx, ny, nz):
+    """periodicZ3D.c leaves Y-list entries unset for ny > nz, and for ny >= nx + 2 its
+    x-edge pass reads and writes node j*ny + (nz-1)*nx*ny outside the lattice."""
+    spec = CaseSpec(lbmodel=1, nx=nx, ny=ny, nz=nz, btype=4, f_model=2, f_Lambda=(1.0,) * 15, g_model=None)
+    with pytest.raises(M.MplbError) as e:
+        M.Case(make_case_dir(spec, root=str(tmp_path)), device=-1)
+    assert "Ny <= Nz" in e.value.msg
+    o = O.Oracle(oracle_params(spec))
+    assert o.error()[0] != 0
