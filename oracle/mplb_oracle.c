@@ -5,28 +5,36 @@
  * reference's evaluation order in every expression (no FMA: build with
  * -ffp-contract=off).  Paths below are relative to the reference's src/.
  *
- * Pinning (what checks this restatement, tests/test_oracle_*.py):
+ * Pinning (what checks this restatement, tests/test_oracle_*.py, tests/test_tables.py):
  *   lattice tables      bit-exact against the reference's own latticeModel/D2Q9_*.c and
  *                       D3Q15_*.c compiled into oracle/_ref (oracle/Makefile), plus the
  *                       isotropy identities and M * invM = I.
+ *   topology            neighbour tables and boundary lists bit-exact against the
+ *                       reference's own latticeBox2D/{genericBoundary,periodicX,periodicY,
+ *                       periodicXY}.c and latticeBox3D/{genericBoundary3D,periodicXY3D,
+ *                       periodicZ3D,periodicXYZ3D}.c compiled into oracle/_ref (refbox.c);
+ *                       duplicate-node periodicity also against an independent wrap formula.
  *   EOS / potential     closed forms: VdW critical point, ideal gas, CS/PR limits.
- *   collision models    invariants: mass (m0) conservation, momentum change equal to the
- *                       force, equilibrium fixed point at F = 0, liTempMRT/myMRT energy
- *                       moment change equal to the source; plus brute-force tiny cases
- *                       recomputed independently in the tests.
+ *   liMRT / xuMRT       the textbook second-order equilibrium is a collision fixed point
+ *                       (every equilibrium moment); bounceBack channels: zero momentum on
+ *                       wall nodes (U = F/2rho), exact Poiseuille parabola with
+ *                       nu = cs2 (1/s_nu - 1/2); shear-wave decay at that nu; flat-interface
+ *                       coexistence against the Li-Luo-Li mechanical-stability condition
+ *                       with eps = -16 G sigma (sigma-term); sigma has no effect where the
+ *                       interaction force vanishes (liMRT uses Fi); surface-tension tensors
+ *                       leave flat coexistence unchanged and scale the Laplace jump by
+ *                       (1 - kappa).  Also mass conservation and momentum change = force.
  *   stream              permutation on fully periodic boxes (exact mass conservation,
  *                       inverse of the reverse stream).
- *   topology            duplicate-node periodicity (period N-1) checked against an
- *                       independent wrap formula; boundary-list order and sizes.
  *   interForce          zero on uniform fields, antisymmetry under mirror, smooth-field
  *                       expansion F ~ -G psi cs2-weighted grad psi.
- *   boundaries          bounceBack mass conservation in closed boxes and the
- *                       tangential-average fixed point; fixedT/fixedU reproduce the
- *                       wall value when the neighbour is at equilibrium; fixedU is the
- *                       non-equilibrium extrapolation (Guo, Zheng, Shi 2002) next to
- *                       forced non-equilibrium neighbours; outflow is the implicit-upwind
- *                       convective condition on the outlet face; glibc rand() stream for
- *                       fixedRandomT taken from the host libc.
+ *   boundaries          bounceBack mass conservation in closed boxes and zero wall
+ *                       momentum; fixedT/fixedU reproduce the wall value when the
+ *                       neighbour is at equilibrium; fixedU is the non-equilibrium
+ *                       extrapolation (Guo, Zheng, Shi 2002) next to forced non-equilibrium
+ *                       neighbours; outflow is the implicit-upwind convective condition on
+ *                       the outlet face; glibc rand() stream for fixedRandomT taken from
+ *                       the host libc.
  *   heatSource          vanishes for uniform T, U; divergence/gradient stencils exact on
  *                       linear fields; with myMRT it makes the steady flux rho dT/dy
  *                       uniform across a static density profile.
@@ -35,8 +43,12 @@
  *                       advects at the fixed-point speed of its whole-moment correction.
  *   macroPressure       closed forms on uniform, linear and quadratic potentials, wall and
  *                       corner nodes included.
- * Functions or quirks that none of these fixes are marked "parity unpinned" where they
- * occur below.
+ * Parity unpinned (marked where they occur): liTempMRT's source filter
+ * ss = 1.5 s - 0.5 ss_old; the heat source's time levels (old T and U, new rho); the
+ * fixedRandomT draw order; xuMRT's sigma-term built from the total force (no physical
+ * law separates it from Fi once gravity acts); the sign of liMRTPressure's one-sided
+ * gradient at walls; the spot box's lists (periodicXY3DSpot.c reads dictIO, whose
+ * headers need <mpi.h>, so it is not compiled into oracle/_ref).
  */
 #include "mplb_oracle.h"
 
@@ -850,6 +862,8 @@ static void xumrt_collision(orc_case* c) {
         m_eq[13] = rho * U[0] * U[2];
         m_eq[14] = 0;
         matvec(L->M, &c->f[id * Q], m, Q);
+        /* xuMRTForce.c:18-69: F = Fi + (rho - rho_0) g, also in the sigma-term (parity
+         * unpinned where gravity acts: the sigma-term is pinned with g = 0 only) */
         double F[3];
         for (int j = 0; j < 3; j++) F[j] = c->Fi[id * 3 + j] + (rho - c->rho0) * c->p.g[j];
         double psi = potential(c, rho, c->T[id]);
@@ -1048,6 +1062,8 @@ static void bc_fixedT(orc_case* c, double* fv, int bid, int model, double Tw) {
 
 /* boundary/fixedRandomT/fixedRandomT.c:14-132: srand(pid) on every call (pid = 0 on a
  * single rank), one rand() per missing link while localTStep < psteps.              */
+/* parity unpinned: the draw order (srand(pid) on every call, one rand() per missing
+ * link in list and link order); the generator itself is pinned against the host libc. */
 static void bc_fixedRandomT(orc_case* c, double* fv, int field, int bid, int model, double Tw, double pert, int psteps) {
     const orc_lattice* L = &c->lat;
     const int Q = L->Q;
@@ -1243,6 +1259,8 @@ static double fd_diff(const orc_case* c, const double* q, int stride, int comp, 
     return 0;
 }
 
+/* parity unpinned: the time levels (previous step's T and U, this step's rho) follow
+ * the call order of multiPhasePP.c:176-240 as read. */
 /* heatSource.c:7-202 */
 void orc_heat_source(orc_case* c) {
     if (!c->g_update) return;
@@ -1283,6 +1301,7 @@ void orc_heat_source(orc_case* c) {
         if (model == ORC_MYMRT) {
             c->ss[id] = -lambda * dot / Cv + divU * c->T[id] * (1.0 - dpdT / (c->rho[id] * Cv));
         } else {
+            /* parity unpinned: the filter ss = 1.5 s - 0.5 ss_old (heatSource.c:149-165) */
             double oldss = current == 0 ? 0 : c->ss[id];
             double s = -lambda * dot / Cv + divU * c->T[id] * (1.0 - dpdT / (c->rho[id] * Cv));
             c->ss[id] = s + 0.5 * (s - oldss);
@@ -1331,8 +1350,8 @@ void orc_macro_pressure(orc_case* c) {
                 }
             }
             pr = pr + G * psi * lap / 12;
-            /* the sign of the one-sided gradient at walls is unpinned but reaches no
-             * output: only |grad psi|^2 is used, which the linear-potential pin fixes */
+            /* parity unpinned: the sign of the one-sided gradient at walls; it reaches no
+             * output (only |grad psi|^2 is used, which the linear-potential pin fixes) */
             double grad[3] = {0, 0, 0};
             if (c->lat.model == ORC_D2Q9) {
                 for (int ax = 0; ax < 2; ax++) {

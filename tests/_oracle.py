@@ -11,7 +11,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+# MPLB_ORACLE_LIB points the tests at another build of the oracle (e.g. one with a
+# deliberately planted mistake, to check that the pins catch it)
+LIB = os.environ.get("MPLB_ORACLE_LIB") or os.path.join(ORACLE_DIR, "liboracle.so")
 REF_TABLES = os.path.join(ORACLE_DIR, "_ref", "libreftables.so")
 
 D2Q9, D3Q15 = 0, 1
