@@ -55,19 +55,14 @@ def oracle_params(spec: CaseSpec, ht=True, frozen=False, nthreads=0):
 
 
 def splitmix64_uniform(seed, n):
-    """Uniform doubles in [0, 1) from splitmix64 (seeded, platform independent)."""
-    out = np.empty(n)
-    x = np.uint64(seed)
-    c1, c2, c3 = np.uint64(0x9E3779B97F4A7C15), np.uint64(0xBF58476D1CE4E5B9), np.uint64(0x94D049BB133111EB)
+    """Uniform doubles in [0, 1) from splitmix64 (seeded, platform independent): element
+    i mixes seed + (i + 1) * 0x9E3779B97F4A7C15, as the sequential generator would."""
     with np.errstate(over="ignore"):
-        for i in range(n):
-            x = x + c1
-            z = x
-            z = (z ^ (z >> np.uint64(30))) * c2
-            z = (z ^ (z >> np.uint64(27))) * c3
-            z = z ^ (z >> np.uint64(31))
-            out[i] = float(z >> np.uint64(11)) * (1.0 / 9007199254740992.0)
-    return out
+        z = np.uint64(seed) + np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
 
 def coords(spec):
