@@ -68,12 +68,13 @@ def make_case(n, thermal, steps_total):
     return d, spec, rho.ravel(), T, (rv, rl)
 
 
-def run_gpu(args):
+def run_gpu(args, thermal=None):
     import multiphaselb_amd as M
 
+    thermal = args.thermal if thermal is None else thermal
     prec = {"fp64": M.FP64, "fp32": M.FP32, "fp16s": M.FP16S}[args.precision]
-    d, spec, rho, T, dens = make_case(args.size, args.thermal, args.steps + args.warmup)
-    case = M.Case(d, precision=prec, heat_transfer=args.thermal, fused=not args.generic,
+    d, spec, rho, T, dens = make_case(args.size, thermal, args.steps + args.warmup)
+    case = M.Case(d, precision=prec, heat_transfer=thermal, fused=not args.generic,
                   extensions=M.EXT_LIMRT_D3Q15)
     N = case.N
     case.set("rho", rho)
@@ -92,9 +93,19 @@ def run_gpu(args):
                 rho_range=(float(rho_end.min()), float(rho_end.max())))
 
 
-def cpu_baseline(args, budget_s=12.0):
-    """The oracle (reference-faithful FP64 C restatement, OpenMP over nodes) on the
-    host cores: D3Q15 xuMRT bubble at 64^3, as many steps as fit the time budget."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_run(args, threads, budget_s):
+    """The oracle on `threads` host threads: D3Q15 bubble at 64^3, as many steps as fit
+    the time budget; returns (MLUPS, steps, seconds)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
     from _cases import oracle_params
@@ -102,23 +113,38 @@ def cpu_baseline(args, budget_s=12.0):
     n = 64
     d, spec, rho, T, _ = make_case(n, args.thermal, 0)
     spec.f_model = 2        # the D3Q15 MRT scheme liMRT is mapped onto
-    threads = os.cpu_count() or 1
-    cap = int(os.environ.get("OMP_NUM_THREADS", threads))
-    threads = max(1, min(threads, cap))
     o = O.Oracle(oracle_params(spec, ht=args.thermal, nthreads=threads))
     N = n ** 3
     o.set("rho", rho)
     o.set("U", np.zeros((N, 3)))
     o.set("T", np.full(N, T))
     o.init()
+    o.run(1)                 # first touch of the arrays outside the timed loop
     steps, t0 = 0, time.perf_counter()
     while time.perf_counter() - t0 < budget_s and steps < 2000:
         o.run(1)
         steps += 1
     dt = time.perf_counter() - t0
-    return dict(value=round(N * steps / dt / 1e6, 3), unit="MLUPS", cores=threads, kind="port",
-                sample="D3Q15 %s bubble 64^3 FP64 oracle, %d steps in %.1f s" %
-                       ("liMRT+myMRT thermal" if args.thermal else "liMRT", steps, dt))
+    return N * steps / dt / 1e6, steps, dt
+
+
+def cpu_baseline(args, budget_s=10.0):
+    """The reference-faithful FP64 C restatement of the step (the oracle: AoS storage,
+    dense mat-vecs, the reference's operation order, OpenMP over nodes) on the host
+    cores, beside the FP32 GPU number: one thread and all the threads the host offers
+    (OMP_NUM_THREADS caps them).  Sample: the same D3Q15 bubble at 64^3."""
+    threads = os.cpu_count() or 1
+    cap = int(os.environ.get("OMP_NUM_THREADS", threads))
+    threads = max(1, min(threads, cap))
+    v1, s1, t1 = cpu_run(args, 1, budget_s)
+    vn, sn, tn = cpu_run(args, threads, budget_s)
+    what = "liMRT+myMRT thermal" if args.thermal else "liMRT"
+    return dict(value=round(vn, 3), unit="MLUPS", cores=threads, kind="port",
+                sample="D3Q15 %s bubble 64^3, FP64 oracle (reference-faithful C, not the FP32 GPU "
+                       "arithmetic): %d steps in %.1f s on %d threads" % (what, sn, tn, threads),
+                cpu_model=cpu_model(),
+                single_thread={"value": round(v1, 3), "unit": "MLUPS", "cores": 1,
+                               "sample": "same case, %d steps in %.1f s" % (s1, t1)})
 
 
 def laplace(rho, p, L, rho_v, rho_l):
@@ -661,6 +687,8 @@ def main():
     ap.add_argument("--thermal", action="store_true")
     ap.add_argument("--generic", action="store_true", help="per-routine kernels instead of the fused kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-thermal-record", action="store_true",
+                    help="default workload: skip the thermal sub-record")
     ap.add_argument("--workload", default="bubble3d", choices=["bubble3d", "ensemble", "warmstart", "accuracy"])
     ap.add_argument("--acc-size", type=int, default=256, help="accuracy: bubble edge (the column is n/2 x n/2 x 3n/2)")
     ap.add_argument("--acc-c2-steps", type=int, default=750, help="accuracy: column steps compared")
@@ -701,11 +729,25 @@ def main():
     if args.workload == "accuracy":
         return run_accuracy(args)
 
-    r = run_gpu(args)
+    out = measure(args, args.thermal)
+    if not args.thermal and not args.no_thermal_record and args.precision != "fp64":
+        # the thermal step (xuMRT + myMRT with heat transfer) on the same bubble, timed the
+        # same way: the configuration the >= 70% roofline target is stated for
+        t = measure(args, True)
+        out["thermal"] = {k: t[k] for k in ("value", "unit", "ms_per_step", "config", "hbm_gbs_effective",
+                                            "step_roofline_frac", "roofline", "kernels_ms", "finite")}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out))
+    return 0
+
+
+def measure(args, thermal):
+    r = run_gpu(args, thermal)
     N, steps = r["N"], args.steps
     sec = r["total_ms"] / 1e3
     mlups = N * steps / sec / 1e6
-    blu = bytes_per_lu(args.precision, args.thermal)
+    blu = bytes_per_lu(args.precision, thermal)
     # Dominant kernel: the fused collide pass (or the collision kernel of the per-routine
     # path).  Its algorithmic bytes per node: populations pulled once and written once,
     # plus the macroscopic values it reads or writes once (psi; thermal: T, S, U).
@@ -715,10 +757,10 @@ def main():
     k_ms, k_launches = r["kern"].get(kname, (0.0, 0))
     # isothermal periodic D3Q15 (FP32 or FP16 storage) runs the single-pass kernel (psi stays
     # on chip): one timed kernel, where the two-pass kernels also report their first pass
-    single = kname == "fused" and not args.thermal and "fused_aux" not in r["kern"]
+    single = kname == "fused" and not thermal and "fused_aux" not in r["kern"]
     if kname == "collide":
         per_unit = 2 * 15 * s_mac + (1 + 3 + 3 + 1) * s_mac      # f in/out; rho, U, Fi, psi in
-    elif args.thermal:
+    elif thermal:
         per_unit = 4 * 15 * s_pdf + (1 + 1 + 1 + 3) * s_mac     # f, g in/out; psi, T, S in; U out
     elif single:
         per_unit = 2 * 15 * s_pdf                                 # f in/out
@@ -729,13 +771,13 @@ def main():
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3 counter
     # runs of this workload (profiles/fused_traffic.json), when one exists for it
     traffic = None
-    key = "d3q15_%d_%s_%s" % (args.size, args.precision, "thermal" if args.thermal else ("iso" if single else "iso_twopass"))
+    key = "d3q15_%d_%s_%s" % (args.size, args.precision, "thermal" if thermal else ("iso" if single else "iso_twopass"))
     tpath = os.path.join(ROOT, "profiles", "fused_traffic.json")
     if kname == "fused" and os.path.exists(tpath):
         rec = json.load(open(tpath)).get("kernels", {}).get(key)
         if rec:
             traffic = rec["hbm_bytes_per_launch"]
-    out = {
+    return {
         "metric": "D3Q15 pseudopot-MRT MLUPS",
         "value": round(mlups, 2),
         "unit": "MLUPS",
@@ -749,7 +791,7 @@ def main():
         "dtype": {"fp64": "f64", "fp32": "f32", "fp16s": "f16-storage/f32"}[args.precision],
         "data": "synthetic (seeded bubble, Maxwell densities %.5f/%.5f)" % r["dens"],
         "config": {"workload": "D3Q15 %s static bubble %d^3 periodic, VdW T=0.8Tc, R=%g" % (
-            "liMRT+myMRT thermal" if args.thermal else "liMRT", args.size, 40.0 * args.size / 256.0),
+            "liMRT+myMRT thermal" if thermal else "liMRT", args.size, 40.0 * args.size / 256.0),
             "precision": args.precision, "path": "fused" if r["fused"] else "per-routine",
             "bytes_per_lu": blu},
         "hbm_gbs_effective": round(step_gbs, 1),
@@ -764,10 +806,6 @@ def main():
         "kernels_ms": {k: round(v[0], 5) for k, v in r["kern"].items()},
         "finite": r["finite"],
     }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args)
-    print(json.dumps(out))
-    return 0
 
 
 if __name__ == "__main__":
