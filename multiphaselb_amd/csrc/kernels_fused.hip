@@ -279,13 +279,14 @@ __global__ void __launch_bounds__(256) k_fused_density(FArgs a) {
     }
 }
 
-// KH: heatSource.c:26-170 with macroTemperature (myMRTTemperature.c:16-30)
+// heatSource.c:26-170 after the stencils: conductivity, dp/dT by central difference and
+// the source from dot = grad T . grad(1/rho) and divU (shared by k_fused_heat and the
+// thermal single pass, so both round alike)
 template <class L>
-__global__ void __launch_bounds__(256) k_fused_heat(FArgs a) {
-    const NodePos p = node_pos<L>(a);
-    if (!p.valid) return;
-    const int64_t id = p.id;
-    const float rho = a.rho[id], Told = a.Told[id];
+__device__ inline float heat_tail(const FArgs& a, float rho, float Told, float dot, float divU) {
+    // multiply-adds fused within an expression only, so every kernel that inlines this
+    // rounds alike
+#pragma clang fp contract(on)
     const float Cv = (float)a.eos.Cv;
     float lambda;
     if (a.gmodel == 1) {
@@ -294,37 +295,57 @@ __global__ void __launch_bounds__(256) k_fused_heat(FArgs a) {
     } else {
         lambda = rho * Cv * (1.0f / a.gl[3] - 0.5f);
     }
+    float dpdT = p_eos<float>(a.eos, rho, 1.1f * Told) - p_eos<float>(a.eos, rho, 0.9f * Told);
+    dpdT = dpdT / (0.2f * Told);
+    return -lambda * dot / Cv + divU * Told * (1.0f - dpdT / (rho * Cv));
+}
+
+// central-difference terms of one axis (heatSource.c:60-80 with both neighbours present)
+__device__ inline void heat_axis(float Tp, float Tm, float rp, float rm, float up, float um, float& dot, float& divU) {
+#pragma clang fp contract(on)
+    const float gT = 0.5f * (Tp - Tm);
+    const float gR = 0.5f * (1.0f / rp - 1.0f / rm);
+    const float dU = 0.5f * (up - um);
+    dot += gT * gR;
+    divU += dU;
+}
+
+// KH: heatSource.c:26-170 with macroTemperature (myMRTTemperature.c:16-30)
+template <class L>
+__global__ void __launch_bounds__(256) k_fused_heat(FArgs a) {
+    const NodePos p = node_pos<L>(a);
+    if (!p.valid) return;
+    const int64_t id = p.id;
+    const float rho = a.rho[id], Told = a.Told[id];
     float dot = 0.f, divU = 0.f;
 #pragma unroll
     for (int ax = 0; ax < L::D; ax++) {
         // a = nb[id][fd_plus] = x + e_ax, b = nb[id][fd_minus] = x - e_ax
         const int64_t na = downstream<L>(a, p, L::fd_minus[ax]);
         const int64_t nbb = downstream<L>(a, p, L::fd_plus[ax]);
-        float gT, gR, dU;
         const float* Uax = a.Uold + ax * a.N;
         if (na != -1 && nbb != -1) {
-            gT = 0.5f * (a.Told[na] - a.Told[nbb]);
-            gR = 0.5f * (1.0f / a.rho[na] - 1.0f / a.rho[nbb]);
-            dU = 0.5f * (Uax[na] - Uax[nbb]);
-        } else if (na == -1 && nbb != -1) {
-            gT = Told - a.Told[nbb];
-            gR = 1.0f / rho - 1.0f / a.rho[nbb];
-            dU = Uax[id] - Uax[nbb];
-        } else if (na != -1 && nbb == -1) {
-            gT = a.Told[na] - Told;
-            gR = 1.0f / a.rho[na] - 1.0f / rho;
-            dU = Uax[na] - Uax[id];
+            heat_axis(a.Told[na], a.Told[nbb], a.rho[na], a.rho[nbb], Uax[na], Uax[nbb], dot, divU);
         } else {
-            gT = 0.f;
-            gR = 0.f;
-            dU = 0.f;
+            float gT, gR, dU;
+            if (na == -1 && nbb != -1) {
+                gT = Told - a.Told[nbb];
+                gR = 1.0f / rho - 1.0f / a.rho[nbb];
+                dU = Uax[id] - Uax[nbb];
+            } else if (na != -1 && nbb == -1) {
+                gT = a.Told[na] - Told;
+                gR = 1.0f / a.rho[na] - 1.0f / rho;
+                dU = Uax[na] - Uax[id];
+            } else {
+                gT = 0.f;
+                gR = 0.f;
+                dU = 0.f;
+            }
+            dot += gT * gR;
+            divU += dU;
         }
-        dot += gT * gR;
-        divU += dU;
     }
-    float dpdT = p_eos<float>(a.eos, rho, 1.1f * Told) - p_eos<float>(a.eos, rho, 0.9f * Told);
-    dpdT = dpdT / (0.2f * Told);
-    const float s = -lambda * dot / Cv + divU * Told * (1.0f - dpdT / (rho * Cv));
+    const float s = heat_tail<L>(a, rho, Told, dot, divU);
     float T;
     float Sv;
     if (a.gmodel == 1) {
@@ -488,6 +509,60 @@ __device__ inline void collide_node(const FArgs& a, float* v, float rho, const f
     populations_add_sym<L, float>(m, v);
 }
 
+// MRT collision of g (myMRTCollision.c:35-212 / liTempMRTCollision.c:31-84) of one node:
+// v holds its pulled populations on entry and the post-collision ones on exit; T and Sv
+// are the node's new temperature and heat source, U its new velocity.
+template <class L>
+__device__ inline void collide_g(const FArgs& a, float* v, float T, float Sv, const float* U) {
+    float n[L::Q], neq[L::Q];
+    moments_sym<L, float>(v, n);
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) neq[q] = 0.f;
+    neq[0] = T;
+    if (a.gmodel == 1) {
+        neq[1] = a.a1 * T;
+        neq[2] = a.a2 * T;
+    } else {
+        neq[1] = -2.f * T;
+        neq[2] = 2.f * T;
+    }
+    neq[3] = T * U[0];
+    neq[4] = -T * U[0];
+    neq[5] = T * U[1];
+    neq[6] = -T * U[1];
+    if (L::D == 3) {
+        neq[7] = T * U[2];
+        neq[8] = -T * U[2];
+    }
+    if (a.gmodel == 1) {
+        // myMRTCollision.c: n <- n - Q (n - neq) + (I - Q/2) Gamma
+        float d[L::Q];
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) d[q] = n[q] - neq[q];
+        float r[L::Q];
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) r[q] = a.gl[q] * d[q];
+        const int np = L::D == 2 ? 2 : 3;
+#pragma unroll
+        for (int pr = 0; pr < np; pr++) {
+            const int i0 = 3 + 2 * pr, i1 = 4 + 2 * pr;
+            r[i0] += a.gl[i1] * (a.gl[i0] / 2.f - 1.f) * d[i1];
+        }
+        // n becomes the moment change, added to v below
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) n[q] = -r[q];
+        n[0] += (1.f - 0.5f * a.gl[0]) * Sv;
+    } else {
+        const float n4 = n[4], n6 = n[6];
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) n[q] = -a.gl[q] * (n[q] - neq[q]);
+        n[0] += Sv;
+        n[3] += (1.f - a.gl[3] / 2.f) * a.gl[4] * n4;
+        n[5] += (1.f - a.gl[5] / 2.f) * a.gl[6] * n6;
+    }
+    populations_add_sym<L, float>(n, v);
+}
+
 // K2
 template <class L, typename S, bool THERMAL>
 __global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
@@ -520,56 +595,8 @@ __global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
     for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, v[q], a.fshift[q]);
 
     if (THERMAL) {
-            pull_g<L, S>(a, p, v);
-        const float T = a.Tnew[id];
-        const float Sv = a.S[id];
-        float n[L::Q], neq[L::Q];
-        moments_sym<L, float>(v, n);
-#pragma unroll
-        for (int q = 0; q < L::Q; q++) neq[q] = 0.f;
-        neq[0] = T;
-        if (a.gmodel == 1) {
-            neq[1] = a.a1 * T;
-            neq[2] = a.a2 * T;
-        } else {
-            neq[1] = -2.f * T;
-            neq[2] = 2.f * T;
-        }
-        neq[3] = T * U[0];
-        neq[4] = -T * U[0];
-        neq[5] = T * U[1];
-        neq[6] = -T * U[1];
-        if (L::D == 3) {
-            neq[7] = T * U[2];
-            neq[8] = -T * U[2];
-        }
-        if (a.gmodel == 1) {
-            // myMRTCollision.c: n <- n - Q (n - neq) + (I - Q/2) Gamma
-            float d[L::Q];
-#pragma unroll
-            for (int q = 0; q < L::Q; q++) d[q] = n[q] - neq[q];
-            float r[L::Q];
-#pragma unroll
-            for (int q = 0; q < L::Q; q++) r[q] = a.gl[q] * d[q];
-            const int np = L::D == 2 ? 2 : 3;
-#pragma unroll
-            for (int pr = 0; pr < np; pr++) {
-                const int i0 = 3 + 2 * pr, i1 = 4 + 2 * pr;
-                r[i0] += a.gl[i1] * (a.gl[i0] / 2.f - 1.f) * d[i1];
-            }
-            // n becomes the moment change, added to v below
-#pragma unroll
-            for (int q = 0; q < L::Q; q++) n[q] = -r[q];
-            n[0] += (1.f - 0.5f * a.gl[0]) * Sv;
-        } else {
-            const float n4 = n[4], n6 = n[6];
-#pragma unroll
-            for (int q = 0; q < L::Q; q++) n[q] = -a.gl[q] * (n[q] - neq[q]);
-            n[0] += Sv;
-            n[3] += (1.f - a.gl[3] / 2.f) * a.gl[4] * n4;
-            n[5] += (1.f - a.gl[5] / 2.f) * a.gl[6] * n6;
-        }
-        populations_add_sym<L, float>(n, v);
+        pull_g<L, S>(a, p, v);
+        collide_g<L>(a, v, a.Tnew[id], a.S[id], U);
 #pragma unroll
         for (int q = 0; q < L::Q; q++) st<S>(a.gout, q * a.P + id, v[q], a.gshift[q]);
         a.Unew[id] = U[0];
@@ -614,17 +641,23 @@ __device__ inline int wrapd(int u, int n) { return u < 0 ? u + (n - 1) : (u > n 
 // populations of node (x, y, z) pulled from x - c_q (duplicate-node periodic wrap) and
 // their sum
 template <class L, typename S>
-__device__ inline float march_pull(const FArgs& a, int x, int y, int z, float* v) {
+__device__ inline float march_pull_from(const FArgs& a, const void* src, const float* shift, int x, int y, int z,
+                                       float* v) {
     const unsigned nx = (unsigned)a.nx, nxy = (unsigned)a.nxy;
     const unsigned X[3] = {(unsigned)wrapd(x - 1, a.nx), (unsigned)x, (unsigned)wrapd(x + 1, a.nx)};
     const unsigned Y[3] = {(unsigned)wrapd(y - 1, a.ny) * nx, (unsigned)y * nx, (unsigned)wrapd(y + 1, a.ny) * nx};
     const unsigned Z[3] = {(unsigned)wrapd(z - 1, a.nz) * nxy, (unsigned)z * nxy, (unsigned)wrapd(z + 1, a.nz) * nxy};
 #pragma unroll
     for (int q = 0; q < L::Q; q++) {
-        const void* plane = (const char*)a.fin + (size_t)q * a.P * sizeof(S);
-        v[q] = ld<S>(plane, X[1 - L::t.c[q][0]] + Y[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]], a.fshift[q]);
+        const void* plane = (const char*)src + (size_t)q * a.P * sizeof(S);
+        v[q] = ld<S>(plane, X[1 - L::t.c[q][0]] + Y[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]], shift[q]);
     }
     return rho_sym<L, float>(v);
+}
+
+template <class L, typename S>
+__device__ inline float march_pull(const FArgs& a, int x, int y, int z, float* v) {
+    return march_pull_from<L, S>(a, a.fin, a.fshift, x, y, z, v);
 }
 
 template <class L, typename S>
@@ -720,6 +753,212 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPLB_M
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Single-pass thermal step (xuMRT f + myMRT g with heat transfer) on fully periodic
+// D3Q15 boxes: the three passes (density, heat source, collide) of multiPhasePP.c:176-240
+// in one z-march, so f and g are pulled from memory once per step.  A workgroup owns an
+// MX x MY column of sites over a chunk of z planes; for collision plane z it
+//   1. pulls f on the column and a two-site ring around it at plane z + 2 (rho), and loads
+//      the previous step's T and U there (LDS rings of three planes),
+//   2. at plane z + 1, on the column and a one-site ring: heat source from the rho, T and U
+//      stencils in LDS, g pulled for its sum, new temperature and psi (LDS ring),
+//   3. collides plane z: f pulled two planes earlier (registers), psi stencil from LDS,
+//      force and velocity, f collision; g pulled again (an L2 hit: it was pulled one plane
+//      earlier), g collision; writes both, the new T and U.
+// Per node and step HBM sees f and g read once and written once and T, U read once and
+// written once.  The arithmetic of every node is that of k_fused_density, k_fused_heat
+// and k_fused_collide (shared device functions): results are bit-identical.
+// Column of TMX x TMY sites, one thread each (64 x 8: 512 threads; the rings around the
+// column hold 304 and 148 sites, one extra site for the first threads).  On the 256^3
+// thermal bubble 64 x 8 ran at 1.48 ms per step, 32 x 8 at 1.50, 32 x 16 at 1.53 and 64 x 4
+// (three f pulls per thread and plane) at 1.86.
+#ifndef MPLB_TMX
+#define MPLB_TMX 64
+#define MPLB_TMY 8
+#endif
+constexpr int TMX = MPLB_TMX, TMY = MPLB_TMY, TNT = TMX * TMY;
+constexpr int TRX = TMX + 4, TRY = TMY + 4, TNR = TRX * TRY;    // rho/T/U planes (two-site ring)
+constexpr int TPX = TMX + 2, TPY = TMY + 2, TNP = TPX * TPY;    // psi planes (one-site ring)
+constexpr int TRING2 = TNR - TNT, TRING1 = TNP - TNT;           // ring sites (64 x 8: 304 and 148)
+static_assert(TRING2 <= TNT && TRING1 <= TNT, "at most one ring site per thread");
+
+#ifndef MPLB_TMARCH_WAVES
+#define MPLB_TMARCH_WAVES 2
+#endif
+
+// ring site r of the two-site ring -> (sx, sy) in the TRX x TRY plane
+__device__ inline void tring2(int r, int& sx, int& sy) {
+    if (r < 2 * TRX) {
+        sy = r / TRX;
+        sx = r % TRX;
+    } else if (r < 2 * TRX + 4 * TMY) {
+        const int rr = r - 2 * TRX;
+        sy = 2 + rr / 4;
+        const int c = rr % 4;
+        sx = c < 2 ? c : TMX + c;
+    } else {
+        const int rr = r - 2 * TRX - 4 * TMY;
+        sy = 2 + TMY + rr / TRX;
+        sx = rr % TRX;
+    }
+}
+
+// ring site r of the one-site ring -> (px, py) in the TPX x TPY plane
+__device__ inline void tring1(int r, int& px, int& py) {
+    if (r < TPX) {
+        py = 0;
+        px = r;
+    } else if (r < TPX + 2 * TMY) {
+        const int rr = r - TPX;
+        py = 1 + rr / 2;
+        px = (rr % 2) ? TPX - 1 : 0;
+    } else {
+        py = TPY - 1;
+        px = r - TPX - 2 * TMY;
+    }
+}
+
+__device__ inline int tslot(int zr) { return (zr + 3) % 3; }
+
+template <class L, typename S, bool OUT>
+__global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_TMARCH_WAVES))) k_fused_tmarch(FArgs a) {
+    __shared__ float s_rho[3][TNR], s_T[3][TNR], s_U[3][3][TNR];
+    __shared__ float s_psi[3][TNP];
+    const unsigned wi = (blockIdx.x & 7u) * (unsigned)a.mper + (blockIdx.x >> 3);
+    if (wi >= (unsigned)a.mitems) return;
+    const int chunk = (int)(wi / (unsigned)a.mtiles), tile = (int)(wi - (unsigned)chunk * (unsigned)a.mtiles);
+    const int x0 = (tile % a.mtx) * TMX, y0 = (tile / a.mtx) * TMY;
+    const int z0 = chunk * a.mzc, z1 = min(z0 + a.mzc, a.nz);
+    const int t = threadIdx.x, tx = t % TMX, ty = t / TMX;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool owner = x < a.nx && y < a.ny;
+    const int64_t N = a.N;
+    // ring sites go to the first threads (whole waves skip the extra work); sites needed
+    // by some owner lie at raw x <= nx + 1, y <= ny + 1 (the period n - 1 wraps them)
+    int r2x = 0, r2y = 0, r1x = 0, r1y = 0;
+    if (t < TRING2) tring2(t, r2x, r2y);
+    if (t < TRING1) tring1(t, r1x, r1y);
+    const bool r2v = t < TRING2 && x0 - 2 + r2x <= a.nx + 1 && y0 - 2 + r2y <= a.ny + 1;
+    const bool r1v = t < TRING1 && x0 - 1 + r1x <= a.nx && y0 - 1 + r1y <= a.ny;
+    const bool own2 = x <= a.nx + 1 && y <= a.ny + 1;
+    const bool own1 = x <= a.nx && y <= a.ny;
+
+    float f0[L::Q], f1[L::Q], f2[L::Q], g0[L::Q], g1[L::Q], tmp[L::Q];
+    float s0 = 0.f, T0 = 0.f, s1 = 0.f, T1 = 0.f;
+
+    // rho of the pulled f, previous T and U of site (sx, sy) of the two-site plane zr
+    auto load_site = [&](int sx, int sy, int zr, float* v) {
+        const int gx = wrapd(x0 - 2 + sx, a.nx), gy = wrapd(y0 - 2 + sy, a.ny), gz = wrapd(zr, a.nz);
+        const float rho = march_pull_from<L, S>(a, a.fin, a.fshift, gx, gy, gz, v);
+        const int64_t id = (int64_t)gx + (int64_t)gy * a.nx + (int64_t)gz * a.nxy;
+        const int sl = tslot(zr), i = sy * TRX + sx;
+        s_rho[sl][i] = rho;
+        s_T[sl][i] = a.Told[id];
+        s_U[sl][0][i] = a.Uold[id];
+        s_U[sl][1][i] = a.Uold[N + id];
+        s_U[sl][2][i] = a.Uold[2 * N + id];
+    };
+    // step 1 for plane zr; the own site's populations go to f2
+    auto load_plane = [&](int zr) {
+        __builtin_amdgcn_s_setprio(MPLB_PULL_PRIO);
+        if (own2) load_site(tx + 2, ty + 2, zr, f2);
+        if (r2v) load_site(r2x, r2y, zr, tmp);
+        __builtin_amdgcn_s_setprio(0);
+    };
+    // heat source, new temperature and psi of site (px, py) of the one-site plane zr; the
+    // pulled g populations are left in v
+    auto psi_site = [&](int px, int py, int zr, float* v, float& sv, float& Tv) {
+        const int sl = tslot(zr), sm = tslot(zr - 1), sp = tslot(zr + 1);
+        const int i = (py + 1) * TRX + px + 1;
+        const float rho = s_rho[sl][i], Told = s_T[sl][i];
+        float dot = 0.f, divU = 0.f;
+        heat_axis(s_T[sl][i + 1], s_T[sl][i - 1], s_rho[sl][i + 1], s_rho[sl][i - 1], s_U[sl][0][i + 1],
+                  s_U[sl][0][i - 1], dot, divU);
+        heat_axis(s_T[sl][i + TRX], s_T[sl][i - TRX], s_rho[sl][i + TRX], s_rho[sl][i - TRX], s_U[sl][1][i + TRX],
+                  s_U[sl][1][i - TRX], dot, divU);
+        heat_axis(s_T[sp][i], s_T[sm][i], s_rho[sp][i], s_rho[sm][i], s_U[sp][2][i], s_U[sm][2][i], dot, divU);
+        const float s = heat_tail<L>(a, rho, Told, dot, divU);
+        const int gx = wrapd(x0 - 1 + px, a.nx), gy = wrapd(y0 - 1 + py, a.ny), gz = wrapd(zr, a.nz);
+        const float sg = march_pull_from<L, S>(a, a.gin, a.gshift, gx, gy, gz, v);
+        const float T = sg + 0.5f * s;
+        s_psi[sl][py * TPX + px] = fpot(a, rho, T);
+        sv = s;
+        Tv = T;
+    };
+    // step 2 for plane zr; the own site's g populations, source and temperature go to g1,
+    // s1, T1
+    auto psi_plane = [&](int zr) {
+        float sd, Td;
+        __builtin_amdgcn_s_setprio(MPLB_PULL_PRIO);
+        if (own1) psi_site(tx + 1, ty + 1, zr, g1, s1, T1);
+        if (r1v) psi_site(r1x, r1y, zr, tmp, sd, Td);
+        __builtin_amdgcn_s_setprio(0);
+    };
+
+    load_plane(z0 - 2);
+    load_plane(z0 - 1);
+    load_plane(z0);
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) f0[q] = f2[q];
+    __syncthreads();
+    psi_plane(z0 - 1);
+    __syncthreads();
+    load_plane(z0 + 1);
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) f1[q] = f2[q];
+    __syncthreads();
+    psi_plane(z0);
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) g0[q] = g1[q];
+    s0 = s1;
+    T0 = T1;
+    __syncthreads();
+    for (int z = z0; z < z1; z++) {
+        load_plane(z + 2);
+        __syncthreads();
+        psi_plane(z + 1);
+        __syncthreads();
+        if (owner) {
+            const float rho = rho_sym<L, float>(f0);
+            float psin[L::Q];
+#pragma unroll
+            for (int q = 0; q < L::Q; q++)
+                psin[q] = s_psi[tslot(z + L::t.c[q][2])][(ty + 1 + L::t.c[q][1]) * TPX + tx + 1 + L::t.c[q][0]];
+            float U[3], Fi[3], Ft[3];
+            collide_node<L>(a, f0, rho, psin, true, true, U, Fi, Ft);
+            const int64_t id = (int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy;
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, f0[q], a.fshift[q]);
+            collide_g<L>(a, g0, T0, s0, U);
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) st<S>(a.gout, q * a.P + id, g0[q], a.gshift[q]);
+            a.Tnew[id] = T0;
+            a.Unew[id] = U[0];
+            a.Unew[N + id] = U[1];
+            a.Unew[2 * N + id] = U[2];
+            if (OUT) {
+                a.S[id] = s0;
+                a.out_rho[id] = rho;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    a.out_U[d * N + id] = U[d];
+                    a.out_Fi[d * N + id] = Fi[d];
+                    a.out_Ft[d * N + id] = Ft[d];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) {
+            f0[q] = f1[q];
+            f1[q] = f2[q];
+            g0[q] = g1[q];
+        }
+        s0 = s1;
+        T0 = T1;
+    }
+}
+
 // Post-stream, post-boundary populations of the current time level (what the
 // reference holds in f.value after updateBoundaries), FP32 SoA.
 template <class L, typename S, bool G>
@@ -769,6 +1008,7 @@ struct Fused {
     int face_table = 0;            // faces whose links differ from the periodic wrap
     float fshift[15] = {0}, gshift[15] = {0};
     bool march = false;            // single-pass kernel (isothermal, fully periodic D3Q15)
+    bool tmarch = false;           // thermal single-pass kernel (xuMRT + myMRT, fully periodic D3Q15)
     bool t_uniform = false;        // the isothermal temperature field is one value (t_const)
     float t_const = 0.f;
     int mtx = 0, mtiles = 0, mitems = 0, mper = 0, mzc = 0;
@@ -870,6 +1110,12 @@ void launch_march(mplb_case* c, const FArgs& a) {
     if constexpr (L::D == 3) k_fused_march<L, S, OUT><<<dim3(8u * (unsigned)a.mper), 256, 0, c->stream>>>(a);
 }
 
+// The thermal single-pass kernel (D3Q15, FP32 or FP16 storage)
+template <class L, typename S, bool OUT>
+void launch_tmarch(mplb_case* c, const FArgs& a) {
+    if constexpr (L::D == 3) k_fused_tmarch<L, S, OUT><<<dim3(8u * (unsigned)a.mper), TNT, 0, c->stream>>>(a);
+}
+
 template <class L, typename S>
 void launch_step(mplb_case* c, FArgs a, bool thermal, bool outputs) {
     const dim3 grid((unsigned)a.nblocks), block(256);
@@ -879,7 +1125,9 @@ void launch_step(mplb_case* c, FArgs a, bool thermal, bool outputs) {
         a.out_Ft = (float*)c->Ft;
         if (!thermal) a.out_U = (float*)c->U;
     }
-    if (thermal) {
+    if (thermal && F(c)->tmarch) {
+        launch_tmarch<L, S, false>(c, a);
+    } else if (thermal) {
         k_fused_density<L, S, true><<<grid, block, 0, c->stream>>>(a);
         k_fused_heat<L><<<grid, block, 0, c->stream>>>(a);
         k_fused_collide<L, S, true><<<grid, block, 0, c->stream>>>(a);
@@ -922,7 +1170,10 @@ void launch_step_timed(mplb_case* c, FArgs a, bool thermal, bool outputs) {
         if (!thermal) a.out_U = (float*)c->U;
     }
     // timer slots: 6 = fused collide (dominant), 7 = density/heat passes
-    if (thermal) {
+    if (thermal && F(c)->tmarch) {
+        Ev e(c, 6);
+        launch_tmarch<L, S, false>(c, a);
+    } else if (thermal) {
         {
             Ev e(c, 7);
             k_fused_density<L, S, true><<<grid, block, 0, c->stream>>>(a);
@@ -1056,7 +1307,11 @@ int materialize(mplb_case* c) {
     a.out_Fi = (float*)c->Fi;
     a.out_Ft = (float*)c->Ft;
     a.out_U = (float*)c->U;
-    if (z->thermal) {
+    if (z->thermal && z->tmarch) {
+        // the step's own kernel with outputs on the last step's inputs: it rewrites fa, ga,
+        // T and U with the values they hold
+        launch_tmarch<L, S, true>(c, a);
+    } else if (z->thermal) {
         k_fused_collide<L, S, true><<<grid, block, 0, c->stream>>>(a);
     } else if (z->march) {
         // the step's own kernel with outputs: it rewrites fa with the same values
@@ -1240,16 +1495,30 @@ int fused_create(mplb_case* c) {
         const bool twopass = env && std::strcmp(env, "twopass") == 0;
         bool any_bnd = false;
         for (int64_t id = 0; id < N && !any_bnd; id++) any_bnd = m.onbnd[id] != 0;
-        z->march = !twopass && m.lbmodel == 1 && !z->thermal && z->face_table == 0 && !any_bnd && m.nx >= 3 &&
-                   m.ny >= 3 && m.nz >= 3;
-        if (z->march) {
-            z->mtx = (m.nx + MX - 1) / MX;
-            z->mtiles = z->mtx * ((m.ny + MY - 1) / MY);
+        const bool periodic3 = !twopass && m.lbmodel == 1 && z->face_table == 0 && !any_bnd && m.nx >= 3 && m.ny >= 3 &&
+                               m.nz >= 3;
+        z->march = periodic3 && !z->thermal;
+        // thermal single pass: xuMRT f with myMRT g (the D3Q15 thermal pair)
+        // (opt-in, MPLB_THERMAL_MARCH=1: it reads f and g once per step but runs slower than
+        // the three passes, 1.48 against 1.38 ms per step on the 256^3 thermal bubble)
+        const char* tenv = std::getenv("MPLB_THERMAL_MARCH");
+        z->tmarch = periodic3 && z->thermal && c->fc[1].mp.model == M_MYMRT && tenv && std::strcmp(tenv, "1") == 0;
+        if (z->march || z->tmarch) {
+            const int tmx = z->tmarch ? TMX : MX, tmy = z->tmarch ? TMY : MY;
+            z->mtx = (m.nx + tmx - 1) / tmx;
+            z->mtiles = z->mtx * ((m.ny + tmy - 1) / tmy);
             // z chunks (of at least 8 planes) per column: the work items run in rounds of
             // `resident` workgroups (CUs x workgroups per CU at the kernel's occupancy), so
             // pick the count that minimises rounds x (planes per chunk + 2 halo planes)
+            // (halo: 2 planes for the isothermal kernel's psi stencil, 4 for the thermal
+            // kernel's rho/T/U stencils of the heat source under the psi stencil)
             int bpc = 0, cus = 0;
-            if (z->half)
+            const int halo = z->tmarch ? 4 : 2;
+            if (z->tmarch && z->half)
+                FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_tmarch<D3Q15, _Float16, false>, TNT, 0));
+            else if (z->tmarch)
+                FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_tmarch<D3Q15, float, false>, TNT, 0));
+            else if (z->half)
                 FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march<D3Q15, _Float16, false>, 256, 0));
             else
                 FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march<D3Q15, float, false>, 256, 0));
@@ -1259,7 +1528,7 @@ int fused_create(mplb_case* c) {
             long long best = -1;
             for (int n = 1; n <= std::max(1, m.nz / 8); n++) {
                 const long long rounds = ((long long)z->mtiles * n + resident - 1) / resident;
-                const long long cost = rounds * ((m.nz + n - 1) / n + 2);
+                const long long cost = rounds * ((m.nz + n - 1) / n + halo);
                 if (best < 0 || cost < best) {
                     best = cost;
                     nchunks = n;

@@ -61,6 +61,8 @@ CASES = {
                            dict(T=0.02, r_in=0.5, r_out=0.2, radius=4.0), False, False, 30),
     "d3q15_bubble_nht": (d3(nx=12, ny=12, nz=12, btype=5, write_interval=50),
                          dict(T=0.85 * TC, r_in=RV, r_out=RL, radius=3.5), False, False, 100),
+    "d3q15_bubble_thermal": (d3(nx=12, ny=12, nz=12, btype=5, write_interval=50),
+                             dict(T=0.85 * TC, r_in=RV, r_out=RL, radius=3.5), True, False, 100),
     "d3q15_thermal_column": (d3(nx=6, ny=6, nz=16, btype=3, g=(0.0, 0.0, -1e-6), rho0=1.0, write_interval=40,
                                 f_bc={"Z0": BC("bounceBack"), "Z1": BC("bounceBack")},
                                 g_bc={"Z0": BC("fixedRandomT", (0.95 * TC,), pert=0.5, duration=50),
