@@ -76,6 +76,19 @@ int mplb_case_open(const char* case_dir, const mplb_opts* opts, mplb_case** out)
 int mplb_case_close(mplb_case* c);
 const char* mplb_last_error(const mplb_case* c);
 int mplb_case_info(const mplb_case* c, mplb_info* out);
+/* The values the case's dictionaries gave, as the library read them (readEOSInfo.c,
+ * readLbeField.c, readTimeInfo.c rules): key "EOS/model" (0 idealGas, 1
+ * Carnahan-Starling, 2 Peng-Robinson, 3 vanDerWaals), "EOS/a", "EOS/b", "EOS/w" (w_pr),
+ * "EOS/R", "EOS/G", "EOS/M", "EOS/Cv", "EOS/rho_0", "EOS/g" (3), "EOS/extForce" (3),
+ * "time/start", "time/end", "time/writeInterval", and per field ("f/" or "g/"):
+ * "Collision" (mplb_info's model codes), "Lambda" (Q), "sigma", "SurfaceTension"
+ * (1 = liSurfTen/xuSurfTen), "kappa_st", "alpha_1", "alpha_2", and "bc/<list>/type"
+ * (0 bounceBack, 1 fixedT, 2 fixedRandomT, 3 fixedGradT, 4 fixedU, 5 uniformHeatFlux,
+ * 6 fixedEOSp, 7 outflow, 8 periodic), "bc/<list>/value" (3), "bc/<list>/pert",
+ * "bc/<list>/duration".  Writes up to n values to out (may be NULL) and returns how many
+ * the key has, or MPLB_E_ARG for an unknown key.  No counterpart in the reference (its
+ * readers keep the values in their structs); it lets a caller check a case's parse. */
+int mplb_case_param(const mplb_case* c, const char* key, double* out, size_t n);
 
 /* Field transfer, double precision host arrays.  name: "rho", "T", "p", "U" (N x 3,
  * node-major as the reference's U[id][j]), "Fi", "Ft", "f", "g" (N x Q, node-major),

@@ -50,6 +50,7 @@ _lib = None
 
 EXPORTS = [
     "mplb_default_opts", "mplb_case_open", "mplb_case_close", "mplb_last_error", "mplb_case_info",
+    "mplb_case_param",
     "mplb_set_field", "mplb_get_field", "mplb_get_neighbours", "mplb_get_boundary", "mplb_init",
     "mplb_step", "mplb_collision", "mplb_stream", "mplb_update_boundaries", "mplb_macro_density",
     "mplb_macro_velocity", "mplb_macro_temperature", "mplb_inter_force", "mplb_heat_source",
@@ -81,6 +82,7 @@ def library():
         "mplb_case_close": ([vp], i),
         "mplb_last_error": ([vp], cp),
         "mplb_case_info": ([vp, C.POINTER(Info)], i),
+        "mplb_case_param": ([vp, C.c_char_p, vp, C.c_size_t], i),
         "mplb_set_field": ([vp, cp, vp, C.c_size_t], i),
         "mplb_get_field": ([vp, cp, vp, C.c_size_t], i),
         "mplb_get_neighbours": ([vp, vp, C.c_size_t], i),
@@ -203,6 +205,15 @@ class Case:
         inf = Info()
         self._check(self.L.mplb_case_info(self.h, C.byref(inf)))
         return inf
+
+    def param(self, key):
+        """Values the dictionaries gave for key (mplb_case_param), as a float array."""
+        n = self.L.mplb_case_param(self.h, key.encode(), None, 0)
+        if n < 0:
+            raise MplbError(n, "unknown parameter key %s" % key)
+        out = np.zeros(max(n, 1))
+        self.L.mplb_case_param(self.h, key.encode(), out.ctypes.data, n)
+        return out[:n]
 
     def _shape(self, name):
         if name in ("rho", "T", "p", "ss"):

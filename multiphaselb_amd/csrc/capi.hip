@@ -116,6 +116,7 @@ int read_eos(mplb_case* c, const Dict& mp) {
     mp.scalar("EOS/M", 0, &e.M);
     mp.scalar("EOS/Cv", 0, &e.Cv);
     mp.scalar("EOS/rho_0", 0, &c->rho0);
+    c->rho0_dict = c->rho0;
     std::vector<double> v;
     if (mp.vector("EOS/g", &v))
         for (int j = 0; j < 3 && j < (int)v.size(); j++) e.g[j] = v[j];
@@ -834,6 +835,62 @@ int mplb_case_info(const mplb_case* c, mplb_info* o) {
     o->fused_active = c->fused ? 1 : 0;
     o->rho0 = c->rho0;
     return 0;
+}
+
+int mplb_case_param(const mplb_case* c, const char* key, double* out, size_t n) {
+    if (!c || !key) return MPLB_E_ARG;
+    std::vector<double> v;
+    const std::string k = key;
+    const EosParams& e = c->eos;
+    auto field = [&](const std::string& name) -> const FieldCfg* {
+        if (name == "f") return &c->fc[0];
+        if (name == "g") return c->fc[1].present ? &c->fc[1] : nullptr;
+        return nullptr;
+    };
+    if (k == "EOS/model") v = {(double)e.type};
+    else if (k == "EOS/a") v = {e.a};
+    else if (k == "EOS/b") v = {e.b};
+    else if (k == "EOS/w") v = {e.w_pr};
+    else if (k == "EOS/R") v = {e.R};
+    else if (k == "EOS/G") v = {e.G};
+    else if (k == "EOS/M") v = {e.M};
+    else if (k == "EOS/Cv") v = {e.Cv};
+    else if (k == "EOS/rho_0") v = {c->rho0_dict};
+    else if (k == "EOS/g") v = {e.g[0], e.g[1], e.g[2]};
+    else if (k == "EOS/extForce") v = {e.fext[0], e.fext[1], e.fext[2]};
+    else if (k == "time/start") v = {(double)c->start};
+    else if (k == "time/end") v = {(double)c->end};
+    else if (k == "time/writeInterval") v = {(double)c->write_interval};
+    else {
+        const size_t sl = k.find('/');
+        const FieldCfg* fc = sl == std::string::npos ? nullptr : field(k.substr(0, sl));
+        if (!fc) return MPLB_E_ARG;
+        const std::string rest = k.substr(sl + 1);
+        if (rest == "Collision") v = {(double)fc->mp.model};
+        else if (rest == "Lambda") v.assign(fc->mp.Lambda, fc->mp.Lambda + c->mesh.Q);
+        else if (rest == "sigma") v = {fc->mp.sigma};
+        else if (rest == "SurfaceTension") v = {(double)fc->mp.surf};
+        else if (rest == "kappa_st") v = {fc->mp.kappa};
+        else if (rest == "alpha_1") v = {fc->mp.alpha1};
+        else if (rest == "alpha_2") v = {fc->mp.alpha2};
+        else if (rest.compare(0, 3, "bc/") == 0) {
+            // bc/<list index>/<type|value|pert|duration>
+            const size_t s2 = rest.find('/', 3);
+            if (s2 == std::string::npos) return MPLB_E_ARG;
+            const int b = std::atoi(rest.substr(3, s2 - 3).c_str());
+            if (b < 0 || b >= (int)fc->bctype.size()) return MPLB_E_ARG;
+            const std::string what = rest.substr(s2 + 1);
+            if (what == "type") v = {(double)fc->bctype[b]};
+            else if (what == "value") v = {fc->bcval[b][0], fc->bcval[b][1], fc->bcval[b][2]};
+            else if (what == "pert") v = {fc->pert[b]};
+            else if (what == "duration") v = {(double)fc->psteps[b]};
+            else return MPLB_E_ARG;
+        } else {
+            return MPLB_E_ARG;
+        }
+    }
+    for (size_t i = 0; i < v.size() && i < n && out; i++) out[i] = v[i];
+    return (int)v.size();
 }
 
 int mplb_set_field(mplb_case* c, const char* name, const double* host, size_t n) {

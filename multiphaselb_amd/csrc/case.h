@@ -47,6 +47,7 @@ struct mplb_case {
     mplb::Mesh mesh;
     mplb::EosParams eos{};
     double rho0 = 0;
+    double rho0_dict = 0;                       // EOS/rho_0 as the dictionary gives it
     mplb::FieldCfg fc[2];
     int update[2] = {1, 1};
     int start = 0, end = 0, write_interval = 1, current = 0, stp = 0;

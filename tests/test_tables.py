@@ -230,6 +230,69 @@ def test_dictionary_reader_on_reference_examples():
     assert c.info.lbmodel == 1
 
 
+# Every value the six shipped example dictionaries give, read off the files by hand.
+_F9 = (1.0, 1.1, 1.1, 1.0, 1.1, 1.0, 1.1, 0.8, 0.8)
+_G9 = (0.0, 1.0, 1.0, 1.2, 1.0, 1.2, 1.0, 1.0, 1.0)
+_F15 = (1.0, 1.1, 1.1, 1.0, 1.1, 1.0, 1.1, 1.0, 1.1, 1.0, 1.0, 1.0, 1.0, 1.0, 1.2)
+_G15 = (0.0, 1.0, 1.0, 1.2, 1.0, 1.2, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0)
+_VDW = {"EOS/model": (3,), "EOS/a": (0.5,), "EOS/b": (4.0,), "EOS/R": (1.0,), "EOS/M": (1.0,), "EOS/G": (-1.0,),
+        "EOS/rho_0": (0.0,), "EOS/Cv": (1.0,), "EOS/extForce": (0.0, 0.0, 0.0)}
+_P, _BB, _FT = 8, 0, 1
+SHIPPED = {
+    "VdWColumn": dict(
+        {**_VDW, "EOS/g": (0.0, -1.234567e-07, 0.0), "time/start": (0,), "time/end": (200000,),
+         "time/writeInterval": (25000,), "f/Collision": (0,), "f/Lambda": _F9, "f/sigma": (0.125,),
+         "f/SurfaceTension": (0,), "g/Collision": (1,), "g/Lambda": _G9, "g/alpha_1": (0.25,), "g/alpha_2": (0.25,)},
+        bc={"f": [(_P,), (_P,), (_BB,), (_BB,)], "g": [(_P,), (_P,), (_FT, 0.03666667), (_FT, 0.03666667)]}),
+    "VdWColumn3D": dict(
+        {**_VDW, "EOS/g": (0.0, -1.234567e-07, 0.0), "time/start": (0,), "time/end": (200000,),
+         "time/writeInterval": (25000,), "f/Collision": (2,), "f/Lambda": _F15, "f/sigma": (1.2,),
+         "f/SurfaceTension": (0,), "g/Collision": (1,), "g/Lambda": _G15, "g/alpha_1": (0.25,), "g/alpha_2": (0.25,)},
+        # periodicXY in 3D has the Z0 and Z1 lists only; the X and Y entries are never read
+        bc={"f": [(_BB,), (_BB,)], "g": [(_FT, 0.03666667), (_FT, 0.03666667)]}),
+    "VdWColumn_HT": dict(
+        {**_VDW, "EOS/g": (0.0, -1.234567e-07, 0.0), "time/start": (0,), "time/end": (1000000,),
+         "time/writeInterval": (100000,), "f/Collision": (0,), "f/Lambda": _F9, "f/sigma": (0.125,),
+         "f/SurfaceTension": (0,), "g/Collision": (1,), "g/Lambda": (0.0, 1.0, 1.0, 0.8, 1.0, 0.8, 1.0, 1.0, 1.0),
+         "g/alpha_1": (1.0,), "g/alpha_2": (1.0,)},
+        bc={"f": [(_P,), (_P,), (_BB,), (_BB,)], "g": [(_P,), (_P,), (_FT, 0.033333), (_FT, 0.036667)]}),
+    "maxwell": dict(
+        {**_VDW, "EOS/g": (0.0, 0.0, 0.0), "time/start": (0,), "time/end": (5000,), "time/writeInterval": (500,),
+         "f/Collision": (0,), "f/Lambda": _F9, "f/sigma": (0.125,), "f/SurfaceTension": (0,), "g/Collision": (1,),
+         "g/Lambda": _G9, "g/alpha_1": (0.25,), "g/alpha_2": (0.25,)},
+        bc={"f": [], "g": []}),
+    "singleBubble": dict(
+        {**_VDW, "EOS/g": (0.0, 0.0, 0.0), "time/start": (0,), "time/end": (5000,), "time/writeInterval": (2500,),
+         "f/Collision": (0,), "f/Lambda": (1.0, 0.8, 1.1, 1.0, 1.1, 1.0, 1.1, 0.8, 0.8), "f/sigma": (0.125,),
+         "f/SurfaceTension": (0,), "g/Collision": (1,), "g/Lambda": _G9, "g/alpha_1": (0.25,), "g/alpha_2": (0.25,)},
+        bc={"f": [], "g": []}),
+    "singleBubble3D": dict(
+        {**_VDW, "EOS/g": (0.0, 0.0, 0.0), "time/start": (0,), "time/end": (5000,), "time/writeInterval": (100,),
+         "f/Collision": (2,), "f/Lambda": _F15, "f/sigma": (0.125,), "f/SurfaceTension": (0,), "g/Collision": (1,),
+         "g/Lambda": _G15, "g/alpha_1": (0.25,), "g/alpha_2": (0.25,)},
+        bc={"f": [], "g": []}),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SHIPPED))
+def test_every_value_of_the_shipped_dictionaries(name):
+    """The library reads every value of each of the reference's six example case
+    dictionaries: EOS block, f and g model constants, times and the boundary conditions
+    list by list (periodicXY boxes have no lists)."""
+    want = dict(SHIPPED[name])
+    bc = want.pop("bc")
+    c = _parse(name)
+    for key, vals in want.items():
+        got = c.param(key)
+        assert np.array_equal(got, np.array(vals, float)), (key, got, vals)
+    for fld in ("f", "g"):
+        assert c.info.nbd == len(bc[fld])
+        for b, exp in enumerate(bc[fld]):
+            assert c.param("%s/bc/%d/type" % (fld, b))[0] == exp[0], (fld, b)
+            if len(exp) > 1:
+                assert c.param("%s/bc/%d/value" % (fld, b))[0] == exp[1], (fld, b)
+
+
 def test_dictionary_first_match_rules(tmp_path):
     """Keys match as substrings at their first occurrence (singleEntry.c:14)."""
     spec = CaseSpec(eos=2, a=0.5, b=4.0, w_pr=0.3, g=(0.0, -1e-5, 0.0))
