@@ -55,6 +55,10 @@ class CaseSpec:
     rho0: float = 0.0
     g: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     fext: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    # write R, M, G, g, extForce, rho_0 and Cv before the model line: the reference's
+    # first-substring lookups (dictIO/singleEntry.c:14) then find these keys and not the
+    # "R" of "Peng-Robinson" or the "g" of "Carnahan-Starling"
+    eos_keys_first: bool = False
     # f
     f_model: int = 0
     f_Lambda: Tuple[float, ...] = (1.0, 1.1, 1.1, 1.0, 1.1, 1.0, 1.1, 1.0, 1.0)
@@ -110,16 +114,18 @@ def write_case(path, spec: CaseSpec):
         if spec.g_model == 1:
             mp += ["  alpha_1 %s;" % _fmt(spec.g_alpha1), "", "  alpha_2 %s;" % _fmt(spec.g_alpha2)]
         mp += ["}", ""]
-    mp += ["EOS", "{", "    model %s;" % EOS_NAMES[spec.eos], ""]
+    common = ["    R %s;" % _fmt(spec.R), "", "    M %s;" % _fmt(spec.M), "",
+              "    G %s;" % _fmt(spec.G), "", "    g %s;" % _vec(spec.g), "",
+              "    extForce %s;" % _vec(spec.fext), "", "    rho_0 %s;" % _fmt(spec.rho0), "",
+              "    Cv %s;" % _fmt(spec.Cv)]
+    mp += ["EOS", "{"] + (common + [""] if spec.eos_keys_first else [])
+    mp += ["    model %s;" % EOS_NAMES[spec.eos], ""]
     mp += {3: ["    a_vdw %s;" % _fmt(spec.a), "", "    b_vdw %s;" % _fmt(spec.b)],
            1: ["    a_cs %s;" % _fmt(spec.a), "", "    b_cs %s;" % _fmt(spec.b)],
            2: ["    a_pr %s;" % _fmt(spec.a), "", "    b_pr %s;" % _fmt(spec.b), "",
                "    w_pr %s;" % _fmt(spec.w_pr)],
            0: []}[spec.eos]
-    mp += ["", "    R %s;" % _fmt(spec.R), "", "    M %s;" % _fmt(spec.M), "",
-           "    G %s;" % _fmt(spec.G), "", "    g %s;" % _vec(spec.g), "",
-           "    extForce %s;" % _vec(spec.fext), "", "    rho_0 %s;" % _fmt(spec.rho0), "",
-           "    Cv %s;" % _fmt(spec.Cv), "}"]
+    mp += ([] if spec.eos_keys_first else [""] + common) + ["}"]
     _write(path, "properties/macroProperties", mp)
 
     _write(path, "properties/simulation",

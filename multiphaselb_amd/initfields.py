@@ -43,7 +43,7 @@ def vdw_critical(a=0.5, b=4.0, R=1.0, M=1.0):
 def maxwell(T, eos="vdw", **kw):
     """(rho_vapour, rho_liquid) at temperature T below the critical point."""
     if eos != "vdw":
-        raise NotImplementedError("Maxwell construction implemented for van der Waals")
+        return _maxwell_numeric(T, eos, **kw)
     a, b, R, M = kw.get("a", 0.5), kw.get("b", 4.0), kw.get("R", 1.0), kw.get("M", 1.0)
     rc, tc = vdw_critical(a, b, R, M)
     if not 0 < T < tc:
@@ -73,6 +73,55 @@ def maxwell(T, eos="vdw", **kw):
     ps = optimize.brentq(area, pmin, pmax, xtol=1e-18, rtol=1e-15)
     rv, rl = roots(ps)
     return float(rv), float(rl)
+
+
+def _maxwell_numeric(T, eos, **kw):
+    """Equal pressures and equal areas in specific volume for any EOS of p_eos: the
+    spinodal densities from a scan of dp/drho, the saturation pressure by bisection on
+    the area integral of (p - ps) / rho^2 (scipy quad)."""
+    from scipy import integrate
+
+    kw = dict(kw)
+    b = kw.get("b", 4.0)
+    rmax = {"cs": 4.0 / b, "pr": 1.0 / b}.get(eos, kw.get("M", 1.0) / b) * (1 - 1e-9)
+    p = lambda r: float(p_eos(r, T, eos, **kw))
+    h = 1e-7 * rmax
+    dp = lambda r: (p(r + h) - p(r - h)) / (2 * h)
+    grid = np.linspace(rmax * 1e-6, rmax * (1 - 1e-6), 4001)
+    d = np.array([dp(r) for r in grid])
+    sign = np.nonzero(np.sign(d[:-1]) != np.sign(d[1:]))[0]
+    if len(sign) < 2:
+        raise ValueError("T is not below the critical temperature of this EOS")
+    lo = optimize.brentq(dp, grid[sign[0]], grid[sign[0] + 1])        # vapour-side spinodal
+    hi = optimize.brentq(dp, grid[sign[1]], grid[sign[1] + 1])        # liquid-side spinodal
+
+    def roots(ps):
+        rv = optimize.brentq(lambda r: p(r) - ps, 1e-300, lo, xtol=1e-300, rtol=1e-15)
+        rl = optimize.brentq(lambda r: p(r) - ps, hi, rmax, xtol=1e-16, rtol=1e-15)
+        return rv, rl
+
+    def area(ps):
+        rv, rl = roots(ps)
+        return integrate.quad(lambda r: (p(r) - ps) / (r * r), rv, rl, limit=400, epsabs=0, epsrel=1e-13)[0]
+
+    pmin = max(p(hi) * (1 + 1e-12), p(lo) * 1e-10)
+    pmax = p(lo) * (1 - 1e-12)
+    ps = optimize.brentq(area, pmin, pmax, xtol=1e-18, rtol=1e-15)
+    rv, rl = roots(ps)
+    return float(rv), float(rl)
+
+
+def critical_temperature(eos="vdw", a=0.5, b=4.0, R=1.0, M=1.0, w=0.344):
+    """T_c: 8a/(27bR) (van der Waals, with the reference's M scaling), 0.3773 a/(bR)
+    (Carnahan-Starling, the value the pseudopotential literature uses), and for
+    Peng-Robinson the 0.170151343 a/(bR) of EOS/p_eos.c."""
+    if eos == "vdw":
+        return vdw_critical(a, b, R, M)[1]
+    if eos == "cs":
+        return 0.3773 * a / (b * R)
+    if eos == "pr":
+        return 0.170151343 * a / (b * R)
+    raise ValueError(eos)
 
 
 def splitmix64(seed, n):

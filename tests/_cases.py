@@ -15,6 +15,8 @@ def parsed_eos_quirks(spec):
     """Values the reference's first-substring dictionary lookups produce for keys that
     also occur inside the EOS model name (dictIO/singleEntry.c:14)."""
     R, g = spec.R, tuple(spec.g)
+    if spec.eos_keys_first:         # the keys come before the model name
+        return R, g
     if spec.eos == 1:    # "Carnahan-Starling": EOS/g matches the 'g' of "Starling"
         g = (0.0, spec.a, 0.0)
     if spec.eos == 2:    # "Peng-Robinson": EOS/R matches "Robinson", EOS/g matches "Peng"
