@@ -273,6 +273,69 @@ def run_ensemble(args):
     return 0
 
 
+def accuracy_variants(args, log, run, rel):
+    """The C1 bubble (256^3 periodic, R = 40, 2000 steps) with the model options the
+    configurations leave out: xuSurfTen at kappa = 0.5 (van der Waals, T = 0.8 Tc), and
+    the Carnahan-Starling (a = 1, b = 4) and Peng-Robinson (a = 2/49, b = 2/21,
+    w = 0.344) equations of state at T = 0.85 Tc with their own Maxwell densities.  FP64
+    (per-routine kernels) against FP32 and FP16 storage (fused kernels): bulk densities,
+    Laplace jump, spurious currents; FP32 MLUPS over the last 500 steps."""
+    import multiphaselb_amd as M
+    from multiphaselb_amd.casefiles import CaseSpec
+    from multiphaselb_amd.initfields import critical_temperature, maxwell, noise, sphere
+
+    n, steps = args.acc_size, 2000
+    N = n ** 3
+    c0 = (n - 1) / 2.0
+    R = 40.0 * n / 256.0
+    idx = np.indices((n, n, n)).reshape(3, -1)[::-1]
+    r = np.sqrt(((idx - c0) ** 2).sum(0))
+    inner, outer = r < 0.5 * R, r > R + 20.0 * n / 256.0
+    variants = [("xusurften_kappa0.5", 3, "vdw", dict(a=0.5, b=4.0), 0.8, dict(f_st=1, f_kappa=0.5)),
+                ("carnahan_starling", 1, "cs", dict(a=1.0, b=4.0), 0.85, {}),
+                ("peng_robinson", 2, "pr", dict(a=2.0 / 49.0, b=2.0 / 21.0, w=0.344), 0.85, {})]
+    out = {}
+    for name, code, eos, kw, tr, extra in variants:
+        T = tr * critical_temperature(eos, **kw)
+        rv, rl = maxwell(T, eos, **kw)
+        spec = CaseSpec(lbmodel=1, nx=n, ny=n, nz=n, btype=5, f_model=0, f_Lambda=LAM15, f_sigma=0.125, g_model=None,
+                        eos=code, a=kw["a"], b=kw["b"], w_pr=kw.get("w", 0.344), eos_keys_first=True,
+                        end=10 ** 8, write_interval=steps, **extra)
+        d = tempfile.mkdtemp(prefix="mplb_f4_")
+        M.write_case(d, spec)
+        rho = (sphere((n, n, n), (c0, c0, c0), R, rv, rl, 5.0) + noise((n, n, n), 1e-4 * rl, 20240603)).ravel()
+        obs = {}
+        for pname, prec, fused in (("fp64", M.FP64, False), ("fp32", M.FP32, True), ("fp16s", M.FP16S, True)):
+            t0 = time.perf_counter()
+            c = M.Case(d, precision=prec, heat_transfer=False, fused=fused, extensions=M.EXT_LIMRT_D3Q15)
+            c.set("rho", rho)
+            c.set("U", np.zeros((N, 3)))
+            c.set("T", np.full(N, T))
+            c.init()
+            mlups = None
+            if pname == "fp32":
+                c.step(steps - 500)
+                done, ms, _ = c.time_steps(500)
+                mlups = N * done / (ms / 1e3) / 1e6
+            else:
+                c.step(steps)
+            f = {k: np.asarray(c.get(k)) for k in ("rho", "U", "p")}
+            c.close()
+            um = np.sqrt((f["U"] ** 2).sum(1))
+            obs[pname] = dict(rho_in=float(f["rho"][inner].mean()), rho_out=float(f["rho"][outer].mean()),
+                              dp=float(f["p"][inner].mean() - f["p"][outer].mean()), umax=float(um.max()),
+                              finite=bool(np.isfinite(f["rho"]).all()), wall_s=round(time.perf_counter() - t0, 1))
+            if mlups:
+                obs[pname]["mlups"] = round(mlups, 1)
+            log("F4 %s %s: %s" % (name, pname, obs[pname]))
+        ref = obs["fp64"]
+        out[name] = {"T_over_Tc": tr, "maxwell": [rv, rl], "fp64": ref,
+                     **{k: {**obs[k], "rel_err_vs_fp64": {q: rel(obs[k][q], ref[q]) for q in
+                                                          ("rho_in", "rho_out", "dp", "umax")}}
+                        for k in ("fp32", "fp16s")}}
+    return out
+
+
 def run_to_steady_case(case, rho_l, thr, check_every, max_steps):
     """Steps one case until max|drho|/rho_l over check_every steps < thr; returns
     (steps, wall seconds, steady flag)."""
@@ -636,10 +699,13 @@ def run_accuracy(args):
                             **{q: rel(obs2[k][q], ref2[q]) for q in ("rho_liquid", "rho_vapour", "umax")},
                             "T_profile_max": float(np.max(np.abs(tprof[k] - tprof["fp64"]) / tprof["fp64"]))}}
                            for k in ("fp32", "fp16s")}}
+    # F4: the surface-tension tensor and the other equations of state on the C1 bubble
+    res["f4_variants"] = accuracy_variants(args, log, run, rel)
     out = {"metric": "reduced precision vs FP64 on the configuration sizes", "value": None, "unit": "relative error",
            "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": False,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32, f16-storage/f32 vs f64", "data": "synthetic",
-           "config": {"workload": "C1 bubble %d^3 2000 steps; C2 thermal column %dx%dx%d %d steps, Cv=%g" % (
+           "config": {"workload": "C1 bubble %d^3 2000 steps; C2 thermal column %dx%dx%d %d steps, Cv=%g; "
+                                  "C1 with xuSurfTen, Carnahan-Starling, Peng-Robinson" % (
                n, nx, nx, nz, steps2, args.acc_c2_cv)}, **res}
     print(json.dumps(out))
     return 0

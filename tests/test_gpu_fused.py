@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 
 FUSED_CASES = ["d2q9_droplet_nht", "d2q9_bubble_thermal", "d2q9_column_fixedT", "d2q9_column_liTempMRT",
                "d2q9_surface_tension", "d3q15_bubble_nht", "d3q15_thermal_column", "d3q15_surface_tension",
-               "d2q9_column_two_frdt", "d3q15_column_frdt_nopert", "d3q15_bubble_thermal"]
+               "d2q9_column_two_frdt", "d3q15_column_frdt_nopert", "d3q15_bubble_thermal",
+               "d2q9_carnahan_starling_keys_first", "d3q15_peng_robinson_keys_first"]
 
 
 def rel(a, b):

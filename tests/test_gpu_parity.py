@@ -9,6 +9,7 @@ import pytest
 import multiphaselb_amd as M
 from _cases import max_rel, pair, sphere_fields
 from multiphaselb_amd.casefiles import BC, CaseSpec
+from multiphaselb_amd.initfields import critical_temperature, maxwell
 
 pytestmark = pytest.mark.gpu
 
@@ -19,6 +20,10 @@ GLAM9 = (0.0, 1.0, 1.0, 1.2, 1.0, 1.2, 1.0, 1.0, 1.0)
 GLAM15 = (0.0, 1.0, 1.0, 1.2, 1.0, 1.2, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0)
 RL, RV = 0.16, 0.03
 TOL = 1e-12
+TC_CS = critical_temperature("cs", a=1.0, b=4.0)
+TC_PR = critical_temperature("pr", a=2.0 / 49.0, b=2.0 / 21.0)
+CS_85 = maxwell(0.85 * TC_CS, "cs", a=1.0, b=4.0)
+PR_85 = maxwell(0.85 * TC_PR, "pr", a=2.0 / 49.0, b=2.0 / 21.0, w=0.344)
 
 
 def d2(**kw):
@@ -59,6 +64,17 @@ CASES = {
                                dict(T=0.02, r_in=0.5, r_out=0.2, radius=4.0), False, False, 30),
     "d2q9_peng_robinson": (d2(nx=16, ny=16, btype=3, eos=2, a=0.02, b=0.4, w_pr=0.3, write_interval=20),
                            dict(T=0.02, r_in=0.5, r_out=0.2, radius=4.0), False, False, 30),
+    # the EOS block with R, g, rho_0 ... ahead of the model name: the first-substring
+    # lookups then read them as written (gravity and buoyancy reference exercised)
+    "d2q9_carnahan_starling_keys_first": (d2(nx=16, ny=20, btype=3, eos=1, a=1.0, b=4.0, g=(0.0, -2e-6, 0.0), rho0=1.0,
+                                             eos_keys_first=True, write_interval=20),
+                                          dict(T=0.85 * TC_CS, r_in=CS_85[1], r_out=CS_85[0], radius=4.5), False,
+                                          False, 60),
+    "d3q15_peng_robinson_keys_first": (d3(nx=10, ny=10, nz=10, btype=5, eos=2, a=2.0 / 49.0, b=2.0 / 21.0,
+                                          w_pr=0.344, R=1.0, g=(0.0, 0.0, -1e-5), rho0=1.0, eos_keys_first=True,
+                                          write_interval=20),
+                                       dict(T=0.85 * TC_PR, r_in=PR_85[0], r_out=PR_85[1], radius=3.0), False,
+                                       False, 60),
     "d3q15_bubble_nht": (d3(nx=12, ny=12, nz=12, btype=5, write_interval=50),
                          dict(T=0.85 * TC, r_in=RV, r_out=RL, radius=3.5), False, False, 100),
     "d3q15_bubble_thermal": (d3(nx=12, ny=12, nz=12, btype=5, write_interval=50),
