@@ -824,7 +824,10 @@ def measure(args, thermal):
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3 counter
     # runs of this workload (profiles/fused_traffic.json), when one exists for it
     traffic = None
-    key = "d3q15_%d_%s_%s" % (args.size, args.precision, "thermal" if thermal else ("iso" if single else "iso_twopass"))
+    staged = os.environ.get("MPLB_MARCH_LDS") == "1" and args.precision == "fp32"
+    march_name = "k_fused_march_lds" if staged else "k_fused_march"
+    key = "d3q15_%d_%s_%s" % (args.size, args.precision,
+                              "thermal" if thermal else ("iso" + ("_staged" if staged else "") if single else "iso_twopass"))
     tpath = os.path.join(ROOT, "profiles", "fused_traffic.json")
     if kname == "fused" and os.path.exists(tpath):
         rec = json.load(open(tpath)).get("kernels", {}).get(key)
@@ -853,8 +856,8 @@ def measure(args, thermal):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None,
-                     "traffic": traffic, "kernel": ("k_fused_march" if single else ("k_fused_collide" if kname == "fused"
-                                                                                                     else "k_collide_*")),
+                     "traffic": traffic, "kernel": (march_name if single else ("k_fused_collide" if kname == "fused"
+                                                                                                else "k_collide_*")),
                      "kernel_ms": round(k_ms, 5),
                      "bytes_per_unit": per_unit, "units_per_launch": N},
         "kernels_ms": {k: round(v[0], 5) for k, v in r["kern"].items()},

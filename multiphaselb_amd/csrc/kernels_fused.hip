@@ -959,6 +959,171 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
     }
 }
 
+// ---------------------------------------------------------------------------
+// The isothermal single pass with its pulls staged in LDS (FP32 storage).  Plane z + 2's
+// populations for the column and its ring go straight from memory into LDS
+// (global_load_lds_dword, no registers held while in flight) and arrive while the
+// workgroup turns plane z + 1's stage into psi and collides plane z.  Two stages of
+// 15 x 396 floats (47.5 KB) and the psi ring (6.3 KB) leave room for three workgroups
+// per CU.  The workgroup meets with a bare s_barrier: a fence would wait for the stage
+// in flight (the LDS writes of the direct loads count as vector-memory operations).  The
+// wait before a stage is read is vmcnt(0), the previous plane's stores included: with
+// loads and stores mixed the vector-memory counter need not count down in issue order.
+constexpr int LST = MX * MY + MRING;       // staged sites: the column, then its ring
+#ifndef MPLB_MARCH_LDS_WAVES
+#define MPLB_MARCH_LDS_WAVES 3
+#endif
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// s_waitcnt immediates: every counter at zero; only the LDS counter at zero
+constexpr int kWaitAll = 0;
+constexpr int kWaitLds = 0xF | 0x7 << 4 | 3 << 14;
+
+// the workgroup meets; the memory clobber keeps the compiler from moving LDS accesses
+// across it
+__device__ inline void wg_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+// direct loads of node (x, y, z)'s pulled populations into stage[q][slot]; the wave's
+// 64 lanes fill 64 consecutive floats from the wave's first slot
+template <class L>
+__device__ inline void stage_pull(const FArgs& a, int x, int y, int z, float (*stage)[LST], int wave_slot) {
+    const unsigned nx = (unsigned)a.nx, nxy = (unsigned)a.nxy;
+    const unsigned X[3] = {(unsigned)wrapd(x - 1, a.nx), (unsigned)x, (unsigned)wrapd(x + 1, a.nx)};
+    const unsigned Y[3] = {(unsigned)wrapd(y - 1, a.ny) * nx, (unsigned)y * nx, (unsigned)wrapd(y + 1, a.ny) * nx};
+    const unsigned Z[3] = {(unsigned)wrapd(z - 1, a.nz) * nxy, (unsigned)z * nxy, (unsigned)wrapd(z + 1, a.nz) * nxy};
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) {
+        const float* src = (const float*)a.fin + (size_t)q * a.P +
+                           (X[1 - L::t.c[q][0]] + Y[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]]);
+        __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)&stage[q][wave_slot], 4, 0, 0);
+    }
+}
+
+template <class L, bool OUT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPLB_MARCH_LDS_WAVES)))
+k_fused_march_lds(FArgs a) {
+    __shared__ float stage0[L::Q][LST], stage1[L::Q][LST];
+    __shared__ float ring[4][MSLOT];
+    const unsigned wi = (blockIdx.x & 7u) * (unsigned)a.mper + (blockIdx.x >> 3);
+    if (wi >= (unsigned)a.mitems) return;
+    const int chunk = (int)(wi / (unsigned)a.mtiles), tile = (int)(wi - (unsigned)chunk * (unsigned)a.mtiles);
+    const int x0 = (tile % a.mtx) * MX, y0 = (tile / a.mtx) * MY;
+    const int z0 = chunk * a.mzc, z1 = min(z0 + a.mzc, a.nz);
+    const int t = threadIdx.x, tx = t % MX, ty = t / MX;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool owner = x < a.nx && y < a.ny;
+    const bool edge = x <= a.nx && y <= a.ny;          // raw n wraps onto site 1
+    // every lane of a wave takes part in a direct load (its LDS slot follows from its lane
+    // number), so threads off the lattice load site 0's populations and never use them
+    const int ex = edge ? wrapd(x, a.nx) : 0, ey = edge ? wrapd(y, a.ny) : 0;
+    int rhx = 0, rhy = 0;
+    if (t < MHX) {
+        rhx = t;
+    } else if (t < 2 * MHX) {
+        rhx = t - MHX;
+        rhy = MHY - 1;
+    } else if (t < 2 * MHX + MY) {
+        rhy = 1 + (t - 2 * MHX);
+    } else {
+        rhx = MHX - 1;
+        rhy = 1 + (t - 2 * MHX - MY);
+    }
+    const bool ring_site = t < MRING && x0 + rhx - 1 <= a.nx && y0 + rhy - 1 <= a.ny;
+    const int rx = ring_site ? wrapd(x0 + rhx - 1, a.nx) : 0, ry = ring_site ? wrapd(y0 + rhy - 1, a.ny) : 0;
+    const int own_slot = t & ~63, ring_slot = MX * MY + (t & ~63);
+    float fc[L::Q], fn[L::Q], tmp[L::Q];
+
+    // temperatures of the staged planes (own and ring site), loaded with their stage:
+    // a load issued after the stage's direct loads would make its first use wait for
+    // all of them (the counter counts down in order)
+    float Te0 = a.Tconst, Tr0 = a.Tconst, Te1 = a.Tconst, Tr1 = a.Tconst;
+    // direct loads of raw plane zr (own site; ring site for threads < MRING)
+    auto issue = [&](int zr, float (*stage)[LST], float& Te, float& Tr) {
+        const int zw = wrapd(zr, a.nz);
+        if (!a.Tuniform) {
+            Te = a.Told[(int64_t)ex + (int64_t)ey * a.nx + (int64_t)zw * a.nxy];
+            if (t < MRING) Tr = a.Told[(int64_t)rx + (int64_t)ry * a.nx + (int64_t)zw * a.nxy];
+        }
+        stage_pull<L>(a, ex, ey, zw, stage, own_slot);
+        if (t < MRING) stage_pull<L>(a, rx, ry, zw, stage, ring_slot);
+    };
+    // psi of raw plane zr from its stage into its ring slot; own populations -> fn
+    auto psi_from = [&](int zr, float (*stage)[LST], float Te, float Tr) {
+        float* sl = ring[(zr - z0 + 1) & 3];
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) fn[q] = stage[q][t];
+        if (edge) sl[(ty + 1) * MHX + tx + 1] = fpot(a, rho_sym<L, float>(fn), Te);
+        if (ring_site) {
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) tmp[q] = stage[q][MX * MY + t];
+            sl[rhy * MHX + rhx] = fpot(a, rho_sym<L, float>(tmp), Tr);
+        }
+    };
+    // this wave's loads and stores complete, then the workgroup meets: every wave's direct
+    // loads have landed and every wave is done reading the stage that is loaded next
+    auto sync_stage = [&]() {
+        __builtin_amdgcn_s_waitcnt(kWaitAll);
+        wg_barrier();
+    };
+    // LDS writes complete, then the workgroup meets; direct loads stay in flight
+    auto sync_ring = [&]() {
+        __builtin_amdgcn_s_waitcnt(kWaitLds);
+        wg_barrier();
+    };
+    auto collide = [&](int z) {
+        if (!owner) return;
+        const float rho = rho_sym<L, float>(fc);
+        float psin[L::Q];
+#pragma unroll
+        for (int q = 0; q < L::Q; q++)
+            psin[q] = ring[(z + L::t.c[q][2] - z0 + 1) & 3][(ty + 1 + L::t.c[q][1]) * MHX + tx + 1 + L::t.c[q][0]];
+        float U[3], Fi[3], Ft[3];
+        collide_node<L>(a, fc, rho, psin, true, true, U, Fi, Ft);
+        const int64_t id = (int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy;
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) st<float>(a.fout, q * a.P + id, fc[q], 0.f);
+        if (OUT) {
+            const int64_t N = a.N;
+            a.out_rho[id] = rho;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                a.out_U[d * N + id] = U[d];
+                a.out_Fi[d * N + id] = Fi[d];
+                a.out_Ft[d * N + id] = Ft[d];
+            }
+        }
+    };
+    // plane zr is staged in stage0 when zr - z0 is odd (z0 - 1, z0 + 1, ...), in stage1
+    // when it is even; the loop takes two planes per trip so that both are named statically
+    issue(z0 - 1, stage0, Te0, Tr0);
+    issue(z0, stage1, Te1, Tr1);
+    sync_stage();
+    psi_from(z0 - 1, stage0, Te0, Tr0);
+    psi_from(z0, stage1, Te1, Tr1);
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
+    sync_ring();                                       // stage0 read by every wave
+    issue(z0 + 1, stage0, Te0, Tr0);
+    for (int z = z0; z < z1; z += 2) {
+        sync_stage();                                  // plane z + 1 in stage0
+        if (z + 2 <= z1) issue(z + 2, stage1, Te1, Tr1);
+        psi_from(z + 1, stage0, Te0, Tr0);
+        sync_ring();
+        collide(z);
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
+        if (z + 1 >= z1) break;
+        sync_stage();                                  // plane z + 2 in stage1
+        if (z + 3 <= z1) issue(z + 3, stage0, Te0, Tr0);
+        psi_from(z + 2, stage1, Te1, Tr1);
+        sync_ring();
+        collide(z + 1);
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
+    }
+}
+
 // Post-stream, post-boundary populations of the current time level (what the
 // reference holds in f.value after updateBoundaries), FP32 SoA.
 template <class L, typename S, bool G>
@@ -1009,6 +1174,7 @@ struct Fused {
     float fshift[15] = {0}, gshift[15] = {0};
     bool march = false;            // single-pass kernel (isothermal, fully periodic D3Q15)
     bool tmarch = false;           // thermal single-pass kernel (xuMRT + myMRT, fully periodic D3Q15)
+    bool march_lds = false;        // the isothermal single pass with LDS-staged pulls (FP32)
     bool t_uniform = false;        // the isothermal temperature field is one value (t_const)
     float t_const = 0.f;
     int mtx = 0, mtiles = 0, mitems = 0, mper = 0, mzc = 0;
@@ -1107,7 +1273,16 @@ FArgs make_args(mplb_case* c) {
 // The single-pass kernel (D3Q15, FP32 or FP16 storage)
 template <class L, typename S, bool OUT>
 void launch_march(mplb_case* c, const FArgs& a) {
-    if constexpr (L::D == 3) k_fused_march<L, S, OUT><<<dim3(8u * (unsigned)a.mper), 256, 0, c->stream>>>(a);
+    if constexpr (L::D == 3) {
+        const dim3 grid(8u * (unsigned)a.mper);
+        if constexpr (std::is_same<S, float>::value) {
+            if (F(c)->march_lds) {
+                k_fused_march_lds<L, OUT><<<grid, 256, 0, c->stream>>>(a);
+                return;
+            }
+        }
+        k_fused_march<L, S, OUT><<<grid, 256, 0, c->stream>>>(a);
+    }
 }
 
 // The thermal single-pass kernel (D3Q15, FP32 or FP16 storage)
@@ -1503,6 +1678,9 @@ int fused_create(mplb_case* c) {
         // the three passes, 1.48 against 1.38 ms per step on the 256^3 thermal bubble)
         const char* tenv = std::getenv("MPLB_THERMAL_MARCH");
         z->tmarch = periodic3 && z->thermal && c->fc[1].mp.model == M_MYMRT && tenv && std::strcmp(tenv, "1") == 0;
+        // LDS-staged pulls for the isothermal single pass, FP32 storage (opt-in, MPLB_MARCH_LDS=1)
+        const char* lenv = std::getenv("MPLB_MARCH_LDS");
+        z->march_lds = z->march && !z->half && lenv && std::strcmp(lenv, "1") == 0;
         if (z->march || z->tmarch) {
             const int tmx = z->tmarch ? TMX : MX, tmy = z->tmarch ? TMY : MY;
             z->mtx = (m.nx + tmx - 1) / tmx;
@@ -1518,6 +1696,8 @@ int fused_create(mplb_case* c) {
                 FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_tmarch<D3Q15, _Float16, false>, TNT, 0));
             else if (z->tmarch)
                 FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_tmarch<D3Q15, float, false>, TNT, 0));
+            else if (z->march_lds)
+                FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march_lds<D3Q15, false>, 256, 0));
             else if (z->half)
                 FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march<D3Q15, _Float16, false>, 256, 0));
             else

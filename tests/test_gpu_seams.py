@@ -4,7 +4,8 @@
   z chunks: a 130 x 12 x 40 periodic bubble has three x tiles (the last one 2 sites wide),
   three column rows in y and five 8-plane z chunks, so ring sites cross tile and chunk
   boundaries everywhere.  It must equal the two-pass fused kernels (same per-node
-  arithmetic) and track the FP64 oracle; FP16 storage likewise against its two passes.
+  arithmetic) and track the FP64 oracle; FP16 storage likewise against its two passes,
+  and the FP32 variant that stages each plane's pulls in LDS likewise.
 * k_fused_tmarch (the thermal D3Q15 single pass: xuMRT + myMRT with heat transfer)
   keeps rho, T and U on a two-site ring and psi on a one-site ring in LDS; on a
   130 x 10 x 24 periodic thermal bubble (three x tiles, a partial y tile, z chunks) it
@@ -34,17 +35,28 @@ SEAM = d3(nx=130, ny=12, nz=40, btype=5, g_model=None, write_interval=10 ** 6)
 SEAM_FIELDS = dict(T=0.8 * TC, r_in=0.03, r_out=0.16, radius=5.0, centre=(63.0, 5.5, 19.5))
 
 
-@pytest.mark.parametrize("precision", [M.FP32, M.FP16S])
-def test_march_across_tiles_and_chunks_equals_two_pass(precision, monkeypatch):
+@pytest.mark.parametrize("precision,staged", [(M.FP32, False), (M.FP32, True), (M.FP16S, False)],
+                         ids=["fp32", "fp32-staged", "fp16s"])
+def test_march_across_tiles_and_chunks_equals_two_pass(precision, staged, monkeypatch):
+    """staged: the variant that loads each plane's pulls straight into LDS
+    (MPLB_MARCH_LDS=1); a read-back half way must leave its trajectory unchanged."""
     steps = 120
-    print("seam case 130x12x40 precision", precision, flush=True)
+    print("seam case 130x12x40 precision", precision, "staged" if staged else "", flush=True)
     fields = sphere_fields(SEAM, seed=21, **SEAM_FIELDS)
     monkeypatch.setenv("MPLB_FUSED_KERNEL", "twopass")
     two, _ = pair(SEAM, fields, precision=precision, ht=False, fused=True)
     monkeypatch.delenv("MPLB_FUSED_KERNEL")
+    if staged:
+        monkeypatch.setenv("MPLB_MARCH_LDS", "1")
     one, o = pair(SEAM, fields, precision=precision, ht=False, fused=True)
     assert one.get_info().fused_active == 1
-    assert two.step(steps) == steps and one.step(steps) == steps
+    assert two.step(steps) == steps
+    if staged:
+        assert one.step(steps // 2) == steps // 2
+        one.get("rho")
+        assert one.step(steps - steps // 2) == steps - steps // 2
+    else:
+        assert one.step(steps) == steps
     for f in ("rho", "U", "f"):
         a, b = one.get(f), two.get(f)
         print("  %s single vs two pass: exact %s, max diff %.2e" % (f, np.array_equal(a, b), np.max(np.abs(a - b))),
