@@ -1071,6 +1071,7 @@ k_fused_march_lds(FArgs a) {
         __builtin_amdgcn_s_waitcnt(kWaitLds);
         wg_barrier();
     };
+    // plane z's collision, in place in fc (its stores wait for the next plane)
     auto collide = [&](int z) {
         if (!owner) return;
         const float rho = rho_sym<L, float>(fc);
@@ -1080,10 +1081,8 @@ k_fused_march_lds(FArgs a) {
             psin[q] = ring[(z + L::t.c[q][2] - z0 + 1) & 3][(ty + 1 + L::t.c[q][1]) * MHX + tx + 1 + L::t.c[q][0]];
         float U[3], Fi[3], Ft[3];
         collide_node<L>(a, fc, rho, psin, true, true, U, Fi, Ft);
-        const int64_t id = (int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy;
-#pragma unroll
-        for (int q = 0; q < L::Q; q++) st<float>(a.fout, q * a.P + id, fc[q], 0.f);
         if (OUT) {
+            const int64_t id = (int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy;
             const int64_t N = a.N;
             a.out_rho[id] = rho;
 #pragma unroll
@@ -1094,34 +1093,44 @@ k_fused_march_lds(FArgs a) {
             }
         }
     };
-    // plane zr is staged in stage0 when zr - z0 is odd (z0 - 1, z0 + 1, ...), in stage1
-    // when it is even; the loop takes two planes per trip so that both are named statically
+    // plane z's collided populations (in fc) to memory
+    auto store_post = [&](int z) {
+        if (!owner) return;
+        const int64_t id = (int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy;
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) st<float>(a.fout, q * a.P + id, fc[q], 0.f);
+    };
+    // Plane zr is staged in stage0 when zr - z0 is odd (z0 - 1, z0 + 1, ...), in stage1
+    // when it is even; the loop takes two planes per trip so that both are named
+    // statically.  A plane's stores are issued right after the next stage's loads, so
+    // both have a whole plane's work to complete before the wait that covers them.
     issue(z0 - 1, stage0, Te0, Tr0);
     issue(z0, stage1, Te1, Tr1);
     sync_stage();
     psi_from(z0 - 1, stage0, Te0, Tr0);
     psi_from(z0, stage1, Te1, Tr1);
-#pragma unroll
-    for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
     sync_ring();                                       // stage0 read by every wave
     issue(z0 + 1, stage0, Te0, Tr0);
     for (int z = z0; z < z1; z += 2) {
         sync_stage();                                  // plane z + 1 in stage0
         if (z + 2 <= z1) issue(z + 2, stage1, Te1, Tr1);
+        if (z > z0) store_post(z - 1);
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
         psi_from(z + 1, stage0, Te0, Tr0);
         sync_ring();
         collide(z);
-#pragma unroll
-        for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
         if (z + 1 >= z1) break;
         sync_stage();                                  // plane z + 2 in stage1
         if (z + 3 <= z1) issue(z + 3, stage0, Te0, Tr0);
+        store_post(z);
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
         psi_from(z + 2, stage1, Te1, Tr1);
         sync_ring();
         collide(z + 1);
-#pragma unroll
-        for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
     }
+    store_post(z1 - 1);
 }
 
 // Post-stream, post-boundary populations of the current time level (what the
