@@ -20,10 +20,13 @@ Other workloads (--workload):
   accuracy   FP32 and FP16 storage against FP64 on the 256^3 bubble and the
              128 x 128 x 384 thermal column: coexistence densities, Laplace jump,
              spurious currents, temperature profile.
+--pattern-probe times the single pass's memory pattern without arithmetic (flat copy,
+pulls, pulls + ring re-pulls) with multiphaselb_amd/stream_probe.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import tempfile
 import time
@@ -768,9 +771,13 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM traffic counters)")
+    ap.add_argument("--pattern-probe", action="store_true",
+                    help="only run the memory-pattern probe of the single pass (streaming ceilings)")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.pattern_probe:
+        return pattern_probe(args)
     if args.gpus != 1:
         print("bench.py runs on one GPU: the whole lattice lives on a single MI355X "
               "(multi-GPU sharding is a design note, not built); got --gpus %d" % args.gpus, file=sys.stderr)
@@ -791,7 +798,44 @@ def main():
                                             "step_roofline_frac", "roofline", "kernels_ms", "finite")}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
+    if out["roofline"]["achieved"]:
+        out["copy_ceiling"] = copy_ceiling(out["roofline"]["achieved"], args.size)
     print(json.dumps(out))
+    return 0
+
+
+def copy_ceiling(achieved_gbs, size):
+    """A flat copy of the 15 population planes of an size^3 lattice (read + write counted;
+    multiphaselb_amd/stream_probe, run as a child process so that its HIP context is its
+    own): the streaming rate the same box reaches without any stencil, next to the
+    dominant kernel's algorithmic rate."""
+    exe = os.path.join(ROOT, "multiphaselb_amd", "stream_probe")
+    if not os.path.exists(exe):
+        return None
+    r = subprocess.run([exe, str(size), "copy"], capture_output=True, text=True, timeout=120)
+    rows = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not rows:
+        return None
+    gbs = rows[0]["GBs"]
+    return {"GBs": gbs, "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4), "bytes": 120 * size ** 3,
+            "kernel_over_copy": round(achieved_gbs / gbs, 4)}
+
+
+def pattern_probe(args):
+    """Runs the memory-pattern probe (multiphaselb_amd/stream_probe, built with the library)
+    on an --size^3 lattice: a flat copy of the 15 population planes, the single pass's pull
+    pattern without and with the ring re-pulls, at 1-3 z chunks; no collision arithmetic."""
+    exe = os.path.join(ROOT, "multiphaselb_amd", "stream_probe")
+    if not os.path.exists(exe):
+        print("stream_probe is not built; run `python -m multiphaselb_amd.build`", file=sys.stderr)
+        return 2
+    r = subprocess.run([exe, str(args.size)], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        print(r.stdout + r.stderr, file=sys.stderr)
+        return r.returncode
+    rows = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+    print(json.dumps({"metric": "memory-pattern ceilings of the D3Q15 single pass", "unit": "GB/s",
+                      "bytes_per_node": 120, "size": args.size, "patterns": rows}))
     return 0
 
 

@@ -76,7 +76,21 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stdout + r.stderr))
     os.replace(tmp, LIB)
+    _build_probe()
     return LIB
+
+
+# The memory-pattern probe (csrc/stream_probe.hip): a stand-alone program that bench.py
+# runs for the streaming ceilings of the single pass's access pattern.
+PROBE = os.path.join(HERE, "stream_probe")
+
+
+def _build_probe():
+    cmd = [HIPCC, "-x", "hip", os.path.join(CSRC, "stream_probe.hip"), "-o", PROBE + ".tmp", "-O3", "-std=c++17"] + HIP_DEV
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("probe build failed: %s\n%s" % (" ".join(cmd), r.stdout + r.stderr))
+    os.replace(PROBE + ".tmp", PROBE)
 
 
 def main():

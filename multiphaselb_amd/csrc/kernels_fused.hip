@@ -768,7 +768,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPLB_M
 //      earlier), g collision; writes both, the new T and U.
 // Per node and step HBM sees f and g read once and written once and T, U read once and
 // written once.  The arithmetic of every node is that of k_fused_density, k_fused_heat
-// and k_fused_collide (shared device functions): results are bit-identical.
+// and k_fused_collide (shared device functions); the compiler may still fuse a
+// multiply-add differently in the two kernels, so results agree to FP32 rounding.
 // Column of TMX x TMY sites, one thread each (64 x 8: 512 threads; the rings around the
 // column hold 304 and 148 sites, one extra site for the first threads).  On the 256^3
 // thermal bubble 64 x 8 ran at 1.48 ms per step, 32 x 8 at 1.50, 32 x 16 at 1.53 and 64 x 4
@@ -1722,6 +1723,11 @@ int fused_create(mplb_case* c) {
                     best = cost;
                     nchunks = n;
                 }
+            }
+            // MPLB_MARCH_CHUNKS=<n> overrides the count (1 <= n <= Nz; for timing sweeps)
+            if (const char* ce = std::getenv("MPLB_MARCH_CHUNKS")) {
+                const int n = std::atoi(ce);
+                if (n >= 1 && n <= m.nz) nchunks = n;
             }
             z->mzc = (m.nz + nchunks - 1) / nchunks;
             nchunks = (m.nz + z->mzc - 1) / z->mzc;
