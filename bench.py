@@ -44,11 +44,13 @@ GLAM15 = (0.0, 1.0, 1.0, 1.2, 1.0, 1.2, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 
 def bytes_per_lu(precision, thermal, q=15):
     """Algorithmic bytes per lattice update: every population read and written once
     and every macroscopic field carried across steps written and read once."""
-    s_pdf = {"fp64": 8, "fp32": 4, "fp16s": 2}[precision]
+    s_pdf = {"fp64": 8, "fp32": 4, "fp16s": 2, "fp16n": 2}[precision]
     s_mac = 8 if precision == "fp64" else 4
+    # FP16N also carries each node's FP16 reference density from step to step
+    s_ref = 2 * 2 if precision == "fp16n" else 0
     if thermal:
-        return 2 * q * s_pdf * 2 + 2 * s_mac * 5
-    return 2 * q * s_pdf + 2 * s_mac * 1
+        return 2 * q * s_pdf * 2 + 2 * s_mac * 5 + s_ref
+    return 2 * q * s_pdf + 2 * s_mac * 1 + s_ref
 
 
 def make_case(n, thermal, steps_total, cv=4.0):
@@ -75,7 +77,7 @@ def run_gpu(args, thermal=None):
     import multiphaselb_amd as M
 
     thermal = args.thermal if thermal is None else thermal
-    prec = {"fp64": M.FP64, "fp32": M.FP32, "fp16s": M.FP16S}[args.precision]
+    prec = {"fp64": M.FP64, "fp32": M.FP32, "fp16s": M.FP16S, "fp16n": M.FP16N}[args.precision]
     d, spec, rho, T, dens = make_case(args.size, thermal, args.steps + args.warmup, args.cv)
     case = M.Case(d, precision=prec, heat_transfer=thermal, fused=not args.generic,
                   extensions=M.EXT_LIMRT_D3Q15)
@@ -190,8 +192,9 @@ def run_ensemble(args):
             hosts[c["L"]] = M.Case(d, heat_transfer=False, device=-1)
     rhos = [ensemble_density(c) for c in ens]
     nodes = sum(c["L"] ** 2 for c in ens)
-    thr = {"fp64": args.steady_fp64, "fp32": args.steady_fp32, "fp16s": args.steady_fp16s}
-    code = {"fp64": M.FP64, "fp32": M.FP32, "fp16s": M.FP16S}
+    thr = {"fp64": args.steady_fp64, "fp32": args.steady_fp32, "fp16s": args.steady_fp16s,
+           "fp16n": args.steady_fp16n}
+    code = {"fp64": M.FP64, "fp32": M.FP32, "fp16s": M.FP16S, "fp16n": M.FP16N}
     precs = ["fp64"] + [x for x in args.ens_precisions.split(",") if x != "fp64"]
     res = {}
     for prec in precs:
@@ -264,7 +267,7 @@ def run_ensemble(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "fp16s": "f16-storage/f32"}[main_prec],
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp16s": "f16-storage/f32", "fp16n": "f16-node-ref-storage/f32"}[main_prec],
         "data": "synthetic (seeded ensemble, Maxwell densities)",
         "config": {"workload": "%d D2Q9 liMRT droplet/bubble cases, L in {64,96,128,192,256}, R in [6, L/4], "
                                "T/Tc in [0.6, 0.95], VdW a=0.5 b=4, seed 7, one batched launch per pass, "
@@ -308,7 +311,8 @@ def accuracy_variants(args, log, run, rel):
         M.write_case(d, spec)
         rho = (sphere((n, n, n), (c0, c0, c0), R, rv, rl, 5.0) + noise((n, n, n), 1e-4 * rl, 20240603)).ravel()
         obs = {}
-        for pname, prec, fused in (("fp64", M.FP64, False), ("fp32", M.FP32, True), ("fp16s", M.FP16S, True)):
+        for pname, prec, fused in (("fp64", M.FP64, False), ("fp32", M.FP32, True), ("fp16s", M.FP16S, True),
+                                   ("fp16n", M.FP16N, True)):
             t0 = time.perf_counter()
             c = M.Case(d, precision=prec, heat_transfer=False, fused=fused, extensions=M.EXT_LIMRT_D3Q15)
             c.set("rho", rho)
@@ -335,7 +339,7 @@ def accuracy_variants(args, log, run, rel):
         out[name] = {"T_over_Tc": tr, "maxwell": [rv, rl], "fp64": ref,
                      **{k: {**obs[k], "rel_err_vs_fp64": {q: rel(obs[k][q], ref[q]) for q in
                                                           ("rho_in", "rho_out", "dp", "umax")}}
-                        for k in ("fp32", "fp16s")}}
+                        for k in ("fp32", "fp16s", "fp16n")}}
     return out
 
 
@@ -627,7 +631,7 @@ def run_accuracy(args):
 
     log = lambda *a: print("#", *a, file=sys.stderr, flush=True)
     tc = vdw_critical(0.5, 4.0)[1]
-    modes = [("fp64", M.FP64, False), ("fp32", M.FP32, True), ("fp16s", M.FP16S, True)]
+    modes = [("fp64", M.FP64, False), ("fp32", M.FP32, True), ("fp16s", M.FP16S, True), ("fp16n", M.FP16N, True)]
 
     def run(d, N, rho, T, prec, fused, ht, steps):
         t0 = time.perf_counter()
@@ -668,7 +672,7 @@ def run_accuracy(args):
     res["c1_bubble"] = {"size": [n, n, n], "steps": steps1, "fp64": ref,
                         **{k: {**obs[k], "rel_err_vs_fp64": {q: rel(obs[k][q], ref[q]) for q in
                                                              ("rho_in", "rho_out", "dp", "umax")}}
-                           for k in ("fp32", "fp16s")}}
+                           for k in ("fp32", "fp16s", "fp16n")}}
     # C2
     nx, nz, steps2 = args.acc_size // 2, (3 * args.acc_size) // 2, args.acc_c2_steps
     rv, rl = maxwell(0.8 * tc, a=0.5, b=4.0)
@@ -701,7 +705,7 @@ def run_accuracy(args):
                         **{k: {**obs2[k], "rel_err_vs_fp64": {
                             **{q: rel(obs2[k][q], ref2[q]) for q in ("rho_liquid", "rho_vapour", "umax")},
                             "T_profile_max": float(np.max(np.abs(tprof[k] - tprof["fp64"]) / tprof["fp64"]))}}
-                           for k in ("fp32", "fp16s")}}
+                           for k in ("fp32", "fp16s", "fp16n")}}
     # F4: the surface-tension tensor and the other equations of state on the C1 bubble
     res["f4_variants"] = accuracy_variants(args, log, run, rel)
     out = {"metric": "reduced precision vs FP64 on the configuration sizes", "value": None, "unit": "relative error",
@@ -735,7 +739,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "fp16s"])
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "fp16s", "fp16n"])
     ap.add_argument("--thermal", action="store_true")
     ap.add_argument("--cv", type=float, default=4.0,
                     help="heat capacity Cv of the thermal workload (the examples' Cv = 1 makes the coupled "
@@ -763,10 +767,11 @@ def main():
     ap.add_argument("--ws-save-data", default="", help="warmstart: save the training profiles (npz)")
     ap.add_argument("--ws-load-data", default="", help="warmstart: add training profiles from npz files")
     ap.add_argument("--cases", type=int, default=1024, help="ensemble: number of cases")
-    ap.add_argument("--ens-precisions", default="fp32,fp16s", help="ensemble: precisions besides fp64")
+    ap.add_argument("--ens-precisions", default="fp32,fp16s,fp16n", help="ensemble: precisions besides fp64")
     ap.add_argument("--steady-fp64", type=float, default=1e-9)
     ap.add_argument("--steady-fp32", type=float, default=5e-6)
     ap.add_argument("--steady-fp16s", type=float, default=1e-4)
+    ap.add_argument("--steady-fp16n", type=float, default=1e-4)
     ap.add_argument("--max-steps", type=int, default=200000)
     ap.add_argument("--tune-steps", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -848,21 +853,22 @@ def measure(args, thermal):
     # Dominant kernel: the fused collide pass (or the collision kernel of the per-routine
     # path).  Its algorithmic bytes per node: populations pulled once and written once,
     # plus the macroscopic values it reads or writes once (psi; thermal: T, S, U).
-    s_pdf = {"fp64": 8, "fp32": 4, "fp16s": 2}[args.precision]
+    s_pdf = {"fp64": 8, "fp32": 4, "fp16s": 2, "fp16n": 2}[args.precision]
     s_mac = 8 if args.precision == "fp64" else 4
     kname = "fused" if "fused" in r["kern"] else "collide"
     k_ms, k_launches = r["kern"].get(kname, (0.0, 0))
     # isothermal periodic D3Q15 (FP32 or FP16 storage) runs the single-pass kernel (psi stays
     # on chip): one timed kernel, where the two-pass kernels also report their first pass
     single = kname == "fused" and not thermal and "fused_aux" not in r["kern"]
+    s_ref = 2 * 2 if args.precision == "fp16n" else 0             # FP16N: node reference in/out
     if kname == "collide":
         per_unit = 2 * 15 * s_mac + (1 + 3 + 3 + 1) * s_mac      # f in/out; rho, U, Fi, psi in
     elif thermal:
-        per_unit = 4 * 15 * s_pdf + (1 + 1 + 1 + 3) * s_mac     # f, g in/out; psi, T, S in; U out
+        per_unit = 4 * 15 * s_pdf + (1 + 1 + 1 + 3) * s_mac + s_ref   # f, g in/out; psi, T, S in; U out
     elif single:
-        per_unit = 2 * 15 * s_pdf                                 # f in/out
+        per_unit = 2 * 15 * s_pdf + s_ref                         # f in/out
     else:
-        per_unit = 2 * 15 * s_pdf + s_mac                         # f in/out; psi in
+        per_unit = 2 * 15 * s_pdf + s_mac + s_ref                 # f in/out; psi in
     achieved = per_unit * N / (k_ms / 1e3) / 1e9 if k_ms > 0 else None
     step_gbs = mlups * 1e6 * blu / 1e9
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3 counter
@@ -888,7 +894,8 @@ def measure(args, thermal):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "fp16s": "f16-storage/f32"}[args.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32", "fp16s": "f16-storage/f32",
+                  "fp16n": "f16-node-ref-storage/f32"}[args.precision],
         "data": "synthetic (seeded bubble, Maxwell densities %.5f/%.5f)" % r["dens"],
         "config": {"workload": "D3Q15 %s static bubble %d^3 periodic, VdW T=0.8Tc, R=%g%s" % (
             "liMRT+myMRT thermal" if thermal else "liMRT", args.size, 40.0 * args.size / 256.0,

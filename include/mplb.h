@@ -35,14 +35,17 @@ extern "C" {
 /* precision of the device state */
 #define MPLB_FP64 0            /* parity mode: reference arithmetic, bit-exact */
 #define MPLB_FP32 1
-#define MPLB_FP16S 2           /* populations stored as FP16 deviations f - w_k rho_ref, FP32 arithmetic */
+#define MPLB_FP16S 2           /* populations stored as FP16 deviations f - w_k rho_ref, g - geq_k(T_ref) (lattice
+                                  means of the initial fields), FP32 arithmetic */
+#define MPLB_FP16N 3           /* as MPLB_FP16S, but f stored as (f - w_k R(x)) / (w_k R(x)), R(x) the node's own
+                                  density when written (per node, FP16): accurate in both phases, 2 B more per node */
 
 /* extension flags (behaviour the reference does not have; all off by default) */
 #define MPLB_EXT_LIMRT_D3Q15 0x1   /* accept Collision liMRT on D3Q15 as the xuMRT D3Q15 scheme */
 #define MPLB_EXT_NO_FIXEDGRADT_ERROR 0x2  /* treat fixedGradT as a no-op instead of the reference's fatal error */
 
 typedef struct mplb_opts {
-    int precision;         /* MPLB_FP64 / MPLB_FP32 / MPLB_FP16S */
+    int precision;         /* MPLB_FP64 / MPLB_FP32 / MPLB_FP16S / MPLB_FP16N */
     int heat_transfer;     /* 1 = default, 0 = multiPhasePP --nht */
     int frozen;            /* 0 = default, 1 = multiPhasePP --frozen */
     int fused;             /* 1: use the fused structured-box kernels when the case allows it */

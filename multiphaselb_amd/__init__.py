@@ -17,7 +17,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # different flags for a comparison); by default the in-tree build is loaded
 LIB_PATH = os.environ.get("MPLB_LIBRARY") or os.path.join(HERE, "libmplb.so")
 
-FP64, FP32, FP16S = 0, 1, 2
+# FP16S: FP16 deviations from one reference density per case; FP16N: FP16 relative
+# deviations from each node's own density (include/mplb.h)
+FP64, FP32, FP16S, FP16N = 0, 1, 2, 3
 EXT_LIMRT_D3Q15 = 0x1
 EXT_NO_FIXEDGRADT_ERROR = 0x2
 MODEL_NAMES = {0: "liMRT", 1: "myMRT", 2: "xuMRT", 3: "liTempMRT", -1: None}

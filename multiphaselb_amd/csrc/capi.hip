@@ -765,6 +765,8 @@ int mplb_case_open(const char* case_dir, const mplb_opts* opts, mplb_case** out)
     else mplb_default_opts(&c->opts);
     c->dir = case_dir;
     *out = c;
+    if (c->opts.precision < MPLB_FP64 || c->opts.precision > MPLB_FP16N)
+        return fail(c, MPLB_E_ARG, "unknown precision " + std::to_string(c->opts.precision));
     int rc = parse_case(c);
     if (rc) return rc;
     // multiPhasePP.c:106,116 (--frozen / --nht disable the f / g updates)
@@ -773,12 +775,13 @@ int mplb_case_open(const char* case_dir, const mplb_opts* opts, mplb_case** out)
     rc = check_models(c);
     if (rc) return rc;
     if (c->opts.device < 0) return 0;   // host-only: dictionaries and lattice tables, no GPU
-    if (c->opts.precision == MPLB_FP16S && !c->opts.fused)
-        return fail(c, MPLB_E_ARG, "FP16S storage is only available with the fused kernels");
+    const bool half = c->opts.precision == MPLB_FP16S || c->opts.precision == MPLB_FP16N;
+    if (half && !c->opts.fused)
+        return fail(c, MPLB_E_ARG, "FP16 storage is only available with the fused kernels");
     if (c->opts.fused) {
         std::string why;
         if (!fused_supported(c, &why)) {
-            if (c->opts.precision == MPLB_FP16S) return fail(c, MPLB_E_ARG, "fused kernels unavailable: " + why);
+            if (half) return fail(c, MPLB_E_ARG, "fused kernels unavailable: " + why);
             c->fused = false;
         } else {
             c->fused = true;

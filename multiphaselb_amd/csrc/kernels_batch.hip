@@ -18,10 +18,13 @@
 //   * an autotuner times the launch shapes (rows per march item, or nodes per workgroup
 //     of the two-pass kernels) and the XCD ordering on the live table.
 // Storage: FP64 (same arithmetic as the per-routine FP64 kernels: bit-identical to the
-// reference restatement), FP32, or FP16 deviations f - w_k rho_ref with FP32 arithmetic.
+// reference restatement), FP32, or FP16 with FP32 arithmetic: deviations f - w_k rho_ref
+// (FP16S) or relative deviations (f - w_k R(x)) / (w_k R(x)), R(x) the node's own density
+// when it was written, kept per node in FP16 (FP16N).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -40,7 +43,7 @@ struct BCase {
     int64_t off;          // first node of the case in the concatenated arrays
     int64_t n;            // nodes
     double T;             // uniform temperature
-    float rho_ref;        // FP16 deviation reference (initial mean density)
+    float rho_ref;        // FP16S deviation reference (initial mean density)
     int state;            // 0 active, 1 converged, 2 diverged (NaN)
 };
 
@@ -57,6 +60,9 @@ struct BArgs {
     void* rho;            // compute type: initial density, then density snapshots
     void* p;              // compute type: pressure output
     unsigned* delta;      // per case: max |rho - snapshot| as float bits
+    // FP16 storage: per-node reference densities of the input and output levels
+    const _Float16* rin;
+    _Float16* rout;
 };
 
 // Work item of the single-pass kernel: rows [y0, y1) of up to four cases of the same
@@ -118,9 +124,63 @@ __device__ inline Pos pos(const BArgs& a, BCase& bc) {
     return p;
 }
 
-template <typename T>
-__device__ inline T shift_of(const BCase& bc, int q) {
-    return T(D2Q9::t.omega[q]) * T(bc.rho_ref);
+// Two FP16 storage modes (FP32 arithmetic in both):
+// * MPLB_FP16S (storage type _Float16): f_q - w_q rho_ref, rho_ref the case's mean initial
+//   density;
+// * MPLB_FP16N (storage type h16n): f_q at node x as its relative deviation
+//   (f_q - w_q R(x)) / (w_q R(x)), R(x) being x's density when the population was written
+//   (rounded to FP16, one array per level beside the populations): each node's deviations
+//   keep the 11-bit mantissa in both phases of a two-phase case.
+struct h16n {
+    _Float16 v;
+};
+template <typename S>
+constexpr bool kNodeRef = std::is_same<S, h16n>::value;
+static_assert(sizeof(h16n) == 2, "FP16 storage element");
+
+constexpr float kRefMin = 6.103515625e-05f;     // smallest normal FP16 (R never rounds to 0)
+
+// FP16 storage: w_q R of the node a population was stored at (FP16N), or the case's shift
+// of population q (FP16S)
+template <typename S, typename T>
+__device__ inline T wref_of(const BArgs& a, const BCase& bc, int64_t node, int q) {
+    if constexpr (kNodeRef<S>) return T(D2Q9::t.omega[q]) * T((float)a.rin[node]);
+    else if constexpr (sizeof(S) == 2) return T(D2Q9::t.omega[q]) * T(bc.rho_ref);
+    else return T(0);
+}
+
+template <typename S, typename T>
+__device__ inline T ldf(const void* p, int64_t i, T wref) {
+    if constexpr (kNodeRef<S>) return T(fmaf((float)((const _Float16*)p)[i], (float)wref, (float)wref));
+    else return ld<S, T>(p, i, wref);
+}
+
+// FP16N: writes node gid's reference for the output level (its density, at least kRefMin;
+// NaN stays NaN) and returns it with its reciprocal; 0 in the other modes
+template <typename S, typename T>
+__device__ inline T put_ref(const BArgs& a, int64_t gid, T rho, T& rinv) {
+    if constexpr (kNodeRef<S>) {
+        const float rf = (float)rho;
+        const float r = (rf >= kRefMin || rf != rf) ? rf : kRefMin;
+        const _Float16 h = (_Float16)r;
+        a.rout[gid] = h;
+        rinv = T(1.f / (float)h);
+        return T((float)h);
+    } else {
+        rinv = T(0);
+        return T(0);
+    }
+}
+
+template <typename S, typename T>
+__device__ inline void stf(const BCase& bc, void* p, int64_t i, int q, T v, T R, T rinv) {
+    if constexpr (kNodeRef<S>) {
+        const float wref = (float)D2Q9::t.omega[q] * (float)R;
+        const float dev = ((float)v - wref) * ((float)(1.0 / D2Q9::t.omega[q]) * (float)rinv);
+        __builtin_nontemporal_store((_Float16)dev, (_Float16*)p + i);
+    } else {
+        st<S, T>(p, i, v, sizeof(S) == 2 ? T(D2Q9::t.omega[q]) * T(bc.rho_ref) : T(0));
+    }
 }
 
 // populations of the current time level: pull-stream of the post-collision buffer
@@ -129,8 +189,8 @@ template <typename S, typename T>
 __device__ inline void pull(const BArgs& a, const BCase& bc, const Pos& p, const void* src, T* v) {
 #pragma unroll
     for (int q = 0; q < 9; q++) {
-        const unsigned l = p.X[1 - D2Q9::t.c[q][0]] + p.Y[1 - D2Q9::t.c[q][1]];
-        v[q] = ld<S, T>(src, q * a.P + p.off + l, shift_of<T>(bc, q));
+        const int64_t node = p.off + p.X[1 - D2Q9::t.c[q][0]] + p.Y[1 - D2Q9::t.c[q][1]];
+        v[q] = ldf<S, T>(src, q * a.P + node, wref_of<S, T>(a, bc, node, q));
     }
 }
 
@@ -283,8 +343,10 @@ __device__ inline void collide_site(const BArgs& a, const BCase& bc, T* v, const
         }
         populations_add_sym<D2Q9, T>(m, v);
     }
+    T rinv;
+    const T R = put_ref<S, T>(a, gid, rho, rinv);
 #pragma unroll
-    for (int q = 0; q < 9; q++) st<S, T>(a.fout, q * a.P + gid, v[q], shift_of<T>(bc, q));
+    for (int q = 0; q < 9; q++) stf<S, T>(bc, a.fout, q * a.P + gid, q, v[q], R, rinv);
 }
 
 template <typename S, typename T, int MODE>
@@ -342,8 +404,8 @@ __global__ void __launch_bounds__(kMarchLanes) k_batch_march(BArgs a, const MIte
                                (unsigned)((yw == ny - 1 ? 1 : yw + 1) * nx)};
 #pragma unroll
         for (int q = 0; q < 9; q++) {
-            const unsigned l = X[1 - D2Q9::t.c[q][0]] + Y[1 - D2Q9::t.c[q][1]];
-            fn[q] = ld<S, T>(a.fin, q * a.P + bc.off + l, shift_of<T>(bc, q));
+            const int64_t node = bc.off + X[1 - D2Q9::t.c[q][0]] + Y[1 - D2Q9::t.c[q][1]];
+            fn[q] = ldf<S, T>(a.fin, q * a.P + node, wref_of<S, T>(a, bc, node, q));
         }
         ring[(yr - it.y0 + 1) & 3][t] = potential<T>(a.eos, density<T>(fn), T(bc.T));
         if (MPLB_BATCH_PRIO) __builtin_amdgcn_s_setprio(0);
@@ -457,6 +519,8 @@ struct mplb_batch {
     size_t sbytes = 4, cbytes = 4;
     void* fa = nullptr;                 // post-collision populations of the current level
     void* fb = nullptr;
+    _Float16* ra = nullptr;             // FP16 storage: per-node references of fa and fb
+    _Float16* rb = nullptr;
     void* psi = nullptr;
     void* rho = nullptr;
     void* p = nullptr;
@@ -609,6 +673,8 @@ void launch_steps(mplb_batch* b, int nsteps) {
     for (int s = 0; s < nsteps; s++) {
         a.fin = b->fa;
         a.fout = b->fb;
+        a.rin = b->ra;
+        a.rout = b->rb;
         if (b->mrows > 0) {
             if (b->nmwork > 0)
                 k_batch_march<S, T><<<(unsigned)b->nmwork, kMarchLanes, 0, b->stream>>>(a, b->d_mwork, b->mper,
@@ -618,6 +684,7 @@ void launch_steps(mplb_batch* b, int nsteps) {
             k_batch_collide<S, T, FROM_PULL><<<grid, block, 0, b->stream>>>(a);
         }
         std::swap(b->fa, b->fb);
+        std::swap(b->ra, b->rb);
     }
 }
 
@@ -656,6 +723,7 @@ int init_t(mplb_batch* b) {
     BArgs a = bargs(b);
     const dim3 grid((unsigned)b->nwork), block((unsigned)b->bsize);
     a.fout = b->fa;
+    a.rout = b->ra;
     k_batch_psi<S, T, FROM_INIT><<<grid, block, 0, b->stream>>>(a);
     k_batch_collide<S, T, FROM_INIT><<<grid, block, 0, b->stream>>>(a);
     BCHK(b, hipGetLastError());
@@ -674,6 +742,7 @@ int fields_t(mplb_batch* b) {
     if (rc) return rc;
     BArgs a = bargs(b);
     a.fin = b->fa;
+    a.rin = b->ra;
     const dim3 grid((unsigned)b->nwork), block((unsigned)b->bsize);
     k_batch_rho_psi<S, T><<<grid, block, 0, b->stream>>>(a);
     k_batch_pressure<T><<<grid, block, 0, b->stream>>>(a);
@@ -696,6 +765,7 @@ int measure_t(mplb_batch* b, bool compare) {
     if (compare) k_fill_u32<<<(nc + 255) / 256, 256, 0, b->stream>>>(b->d_delta, nc, 0u);
     BArgs a = bargs(b);
     a.fin = b->fa;
+    a.rin = b->ra;
     k_batch_measure<S, T><<<(unsigned)b->nwork, (unsigned)b->bsize, 0, b->stream>>>(a, compare ? 1 : 0);
     BCHK(b, hipGetLastError());
     return 0;
@@ -705,6 +775,7 @@ template <typename Fn>
 int bdispatch(mplb_batch* b, Fn fn) {
     if (b->prec == MPLB_FP64) return fn(double{}, double{});
     if (b->prec == MPLB_FP16S) return fn(_Float16{}, float{});
+    if (b->prec == MPLB_FP16N) return fn(h16n{}, float{});
     return fn(float{}, float{});
 }
 
@@ -719,12 +790,12 @@ int mplb_batch_create(mplb_case* const* cases, int n, int precision, int device,
     if (!out) return MPLB_E_ARG;
     *out = nullptr;
     if (!cases || n <= 0) return MPLB_E_ARG;
-    if (precision != MPLB_FP64 && precision != MPLB_FP32 && precision != MPLB_FP16S) return MPLB_E_ARG;
+    if (precision < MPLB_FP64 || precision > MPLB_FP16N) return MPLB_E_ARG;
     mplb_batch* b = new mplb_batch();
     *out = b;
     b->prec = precision;
     b->dev = device;
-    b->sbytes = precision == MPLB_FP64 ? 8 : (precision == MPLB_FP16S ? 2 : 4);
+    b->sbytes = precision == MPLB_FP64 ? 8 : ((precision == MPLB_FP16S || precision == MPLB_FP16N) ? 2 : 4);
     b->cbytes = precision == MPLB_FP64 ? 8 : 4;
     int64_t off = 0;
     for (int i = 0; i < n; i++) {
@@ -771,6 +842,12 @@ int mplb_batch_create(mplb_case* const* cases, int n, int precision, int device,
     BCHK(b, hipMalloc(&b->p, b->cbytes * b->P));
     BCHK(b, hipMemset(b->fa, 0, b->sbytes * 9 * b->P));
     BCHK(b, hipMemset(b->fb, 0, b->sbytes * 9 * b->P));
+    if (b->prec == MPLB_FP16N) {
+        BCHK(b, hipMalloc(&b->ra, sizeof(_Float16) * b->P));
+        BCHK(b, hipMalloc(&b->rb, sizeof(_Float16) * b->P));
+        BCHK(b, hipMemset(b->ra, 0, sizeof(_Float16) * b->P));
+        BCHK(b, hipMemset(b->rb, 0, sizeof(_Float16) * b->P));
+    }
     BCHK(b, hipMalloc(&b->d_cases, sizeof(BCase) * n));
     BCHK(b, hipMalloc(&b->d_delta, sizeof(unsigned) * n));
     return 0;
@@ -1015,7 +1092,7 @@ int mplb_batch_close(mplb_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    void* ptrs[] = {b->fa, b->fb, b->psi, b->rho, b->p, b->d_cases, b->d_work, b->d_mwork, b->d_delta};
+    void* ptrs[] = {b->fa, b->fb, b->ra, b->rb, b->psi, b->rho, b->p, b->d_cases, b->d_work, b->d_mwork, b->d_delta};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (b->stream) (void)hipStreamDestroy(b->stream);

@@ -82,10 +82,13 @@ struct FArgs {
     float* out_Fi;
     float* out_Ft;
     int first_call;                             // liTempMRT heat source S_old = 0
-    float fshift[15], gshift[15];               // FP16 deviation storage shifts
+    float fshift[15], gshift[15];               // FP16 storage shifts (f: FP16S only)
     // single-pass kernel: column tiles in x, tiles per plane, work items, items per XCD,
     // planes per chunk
     int mtx, mtiles, mitems, mper, mzc;
+    // FP16N storage of f: per-node reference densities of the input and output levels
+    const _Float16* rin;
+    _Float16* rout;
 };
 
 template <typename S>
@@ -110,6 +113,75 @@ __device__ inline void st(void* p, int64_t i, float v, float shift) {
         if (MPLB_NT_STORE) __builtin_nontemporal_store(v, (float*)p + i);
         else ((float*)p)[i] = v;
     }
+}
+
+// Two FP16 storage modes for f (FP32 arithmetic in both):
+// * MPLB_FP16S (storage type _Float16): f_q - w_q rho_ref, rho_ref the lattice mean of
+//   the initial density (one shift per population, fshift);
+// * MPLB_FP16N (storage type h16n): f_q at node x as its relative deviation
+//   (f_q - w_q R(x)) / (w_q R(x)), R(x) being x's density when the population was written
+//   (the collision's density, rounded to FP16 and kept per node beside the populations,
+//   one array per level like the populations).  The reference follows each node's own
+//   density and the deviation is scaled by it, so both phases of a two-phase field keep
+//   the 11-bit mantissa (with one shift the light phase's deviations are of the order of
+//   the density jump; unscaled, they would fall among the FP16 subnormals).
+// g keeps one global shift per population in both: its rest equilibrium depends on T only.
+struct h16n {
+    _Float16 v;
+};
+template <typename S>
+constexpr bool kNodeRef = std::is_same<S, h16n>::value;
+static_assert(sizeof(h16n) == 2, "FP16 storage element");
+
+constexpr float kRefMin = 6.103515625e-05f;     // smallest normal FP16 (R never rounds to 0)
+
+// FP16 storage: w_q R of the node a population was stored at (FP16N), or the global
+// shift of population q (FP16S)
+template <class L, typename S>
+__device__ inline float f_wref(const FArgs& a, int64_t node, int q) {
+    if constexpr (kNodeRef<S>) {
+        return (float)L::t.omega[q] * (float)a.rin[node];
+    } else if constexpr (sizeof(S) == 2) {
+        return a.fshift[q];
+    } else {
+        return 0.f;
+    }
+}
+
+// a stored f population; wref from f_wref
+template <typename S>
+__device__ inline float ldf(const void* p, int64_t i, float wref) {
+    if constexpr (kNodeRef<S>) {
+        return fmaf((float)((const _Float16*)p)[i], wref, wref);
+    } else {
+        return ld<S>(p, i, wref);
+    }
+}
+
+// FP16N: writes node id's reference for the output level (the node's density, at least
+// kRefMin; NaN stays NaN) and returns it with its reciprocal; 0 in the other modes
+template <typename S>
+__device__ inline float put_ref(const FArgs& a, int64_t id, float rho, float& rinv) {
+    if constexpr (kNodeRef<S>) {
+        const float r = (rho >= kRefMin || rho != rho) ? rho : kRefMin;
+        const _Float16 h = (_Float16)r;
+        a.rout[id] = h;
+        const float R = (float)h;
+        rinv = 1.f / R;
+        return R;
+    } else {
+        rinv = 0.f;
+        return 0.f;
+    }
+}
+
+// FP16N: stores f population q of a node (reference R and 1 / R = rinv from put_ref)
+template <class L>
+__device__ inline void stf_ref(void* p, int64_t i, int q, float v, float R, float rinv) {
+    const float wref = (float)L::t.omega[q] * R;
+    const float dev = (v - wref) * ((float)(1.0 / L::t.omega[q]) * rinv);
+    if (MPLB_NT_STORE) __builtin_nontemporal_store((_Float16)dev, (_Float16*)p + i);
+    else ((_Float16*)p)[i] = (_Float16)dev;
 }
 
 struct NodePos {
@@ -184,12 +256,19 @@ __device__ inline int64_t downstream(const FArgs& a, const NodePos& p, int q) {
     return (int64_t)(p.X[1 + L::t.c[q][0]] + p.Y[1 + L::t.c[q][1]] + p.Z[1 + L::t.c[q][2]]);
 }
 
-template <class L, typename S>
+// REF: f stored against per-node references (FP16N); otherwise shift holds the storage
+// shifts
+template <class L, typename S, bool REF>
 __device__ inline void pull(const FArgs& a, const NodePos& p, const void* src, const float* shift, float* v) {
 #pragma unroll
     for (int q = 0; q < L::Q; q++) {
         const void* plane = (const char*)src + (size_t)q * a.P * sizeof(S);
-        v[q] = ld<S>(plane, upstream<L>(a, p, q), shift[q]);
+        if constexpr (REF && kNodeRef<S>) {
+            const unsigned up = upstream<L>(a, p, q);
+            v[q] = ldf<S>(plane, up, f_wref<L, S>(a, up, q));
+        } else {
+            v[q] = ld<S>(plane, upstream<L>(a, p, q), shift[q]);
+        }
     }
 }
 
@@ -245,13 +324,13 @@ __device__ inline void fixed_temperature(const FArgs& a, const NodePos& p, float
 
 template <class L, typename S>
 __device__ inline void pull_f(const FArgs& a, const NodePos& p, float* v) {
-    pull<L, S>(a, p, a.fin, a.fshift, v);
+    pull<L, S, kNodeRef<S>>(a, p, a.fin, a.fshift, v);
     if (p.table && a.bcf[p.id] == FB_BBACK) bounce_back<L>(a, p, v);
 }
 
 template <class L, typename S>
 __device__ inline void pull_g(const FArgs& a, const NodePos& p, float* v) {
-    pull<L, S>(a, p, a.gin, a.gshift, v);
+    pull<L, S, false>(a, p, a.gin, a.gshift, v);
     if (!p.table) return;
     const uint8_t bc = a.bcg[p.id];
     const uint8_t code = bc & 0xF;
@@ -591,8 +670,15 @@ __global__ void __launch_bounds__(256) k_fused_collide(FArgs a) {
     }
     float U[3], Fi[3], Ft[3];
     collide_node<L>(a, v, rho, psin, interior, full, U, Fi, Ft);
+    if constexpr (kNodeRef<S>) {
+        float rinv;
+        const float R = put_ref<S>(a, id, rho, rinv);
 #pragma unroll
-    for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, v[q], a.fshift[q]);
+        for (int q = 0; q < L::Q; q++) stf_ref<L>(a.fout, q * a.P + id, q, v[q], R, rinv);
+    } else {
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, v[q], a.fshift[q]);
+    }
 
     if (THERMAL) {
         pull_g<L, S>(a, p, v);
@@ -639,8 +725,8 @@ static_assert(MX * MY == 256 && MRING <= 256, "one thread per column site and pe
 __device__ inline int wrapd(int u, int n) { return u < 0 ? u + (n - 1) : (u > n - 1 ? u - (n - 1) : u); }
 
 // populations of node (x, y, z) pulled from x - c_q (duplicate-node periodic wrap) and
-// their sum
-template <class L, typename S>
+// their sum (REF: f stored against per-node references, FP16N)
+template <class L, typename S, bool REF>
 __device__ inline float march_pull_from(const FArgs& a, const void* src, const float* shift, int x, int y, int z,
                                        float* v) {
     const unsigned nx = (unsigned)a.nx, nxy = (unsigned)a.nxy;
@@ -650,14 +736,19 @@ __device__ inline float march_pull_from(const FArgs& a, const void* src, const f
 #pragma unroll
     for (int q = 0; q < L::Q; q++) {
         const void* plane = (const char*)src + (size_t)q * a.P * sizeof(S);
-        v[q] = ld<S>(plane, X[1 - L::t.c[q][0]] + Y[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]], shift[q]);
+        if constexpr (REF && kNodeRef<S>) {
+            const unsigned up = X[1 - L::t.c[q][0]] + Y[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]];
+            v[q] = ldf<S>(plane, up, f_wref<L, S>(a, up, q));
+        } else {
+            v[q] = ld<S>(plane, X[1 - L::t.c[q][0]] + Y[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]], shift[q]);
+        }
     }
     return rho_sym<L, float>(v);
 }
 
 template <class L, typename S>
 __device__ inline float march_pull(const FArgs& a, int x, int y, int z, float* v) {
-    return march_pull_from<L, S>(a, a.fin, a.fshift, x, y, z, v);
+    return march_pull_from<L, S, kNodeRef<S>>(a, a.fin, a.fshift, x, y, z, v);
 }
 
 template <class L, typename S>
@@ -735,8 +826,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPLB_M
             float U[3], Fi[3], Ft[3];
             collide_node<L>(a, fc, rho, psin, true, true, U, Fi, Ft);
             const int64_t id = (int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy;
+            if constexpr (kNodeRef<S>) {
+                float rinv;
+                const float R = put_ref<S>(a, id, rho, rinv);
 #pragma unroll
-            for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, fc[q], a.fshift[q]);
+                for (int q = 0; q < L::Q; q++) stf_ref<L>(a.fout, q * a.P + id, q, fc[q], R, rinv);
+            } else {
+#pragma unroll
+                for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, fc[q], a.fshift[q]);
+            }
             if (OUT) {
                 const int64_t N = a.N;
                 a.out_rho[id] = rho;
@@ -851,7 +949,7 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
     // rho of the pulled f, previous T and U of site (sx, sy) of the two-site plane zr
     auto load_site = [&](int sx, int sy, int zr, float* v) {
         const int gx = wrapd(x0 - 2 + sx, a.nx), gy = wrapd(y0 - 2 + sy, a.ny), gz = wrapd(zr, a.nz);
-        const float rho = march_pull_from<L, S>(a, a.fin, a.fshift, gx, gy, gz, v);
+        const float rho = march_pull_from<L, S, kNodeRef<S>>(a, a.fin, a.fshift, gx, gy, gz, v);
         const int64_t id = (int64_t)gx + (int64_t)gy * a.nx + (int64_t)gz * a.nxy;
         const int sl = tslot(zr), i = sy * TRX + sx;
         s_rho[sl][i] = rho;
@@ -881,7 +979,7 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
         heat_axis(s_T[sp][i], s_T[sm][i], s_rho[sp][i], s_rho[sm][i], s_U[sp][2][i], s_U[sm][2][i], dot, divU);
         const float s = heat_tail<L>(a, rho, Told, dot, divU);
         const int gx = wrapd(x0 - 1 + px, a.nx), gy = wrapd(y0 - 1 + py, a.ny), gz = wrapd(zr, a.nz);
-        const float sg = march_pull_from<L, S>(a, a.gin, a.gshift, gx, gy, gz, v);
+        const float sg = march_pull_from<L, S, false>(a, a.gin, a.gshift, gx, gy, gz, v);
         const float T = sg + 0.5f * s;
         s_psi[sl][py * TPX + px] = fpot(a, rho, T);
         sv = s;
@@ -929,8 +1027,15 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
             float U[3], Fi[3], Ft[3];
             collide_node<L>(a, f0, rho, psin, true, true, U, Fi, Ft);
             const int64_t id = (int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy;
+            if constexpr (kNodeRef<S>) {
+                float rinv;
+                const float R = put_ref<S>(a, id, rho, rinv);
 #pragma unroll
-            for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, f0[q], a.fshift[q]);
+                for (int q = 0; q < L::Q; q++) stf_ref<L>(a.fout, q * a.P + id, q, f0[q], R, rinv);
+            } else {
+#pragma unroll
+                for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, f0[q], a.fshift[q]);
+            }
             collide_g<L>(a, g0, T0, s0, U);
 #pragma unroll
             for (int q = 0; q < L::Q; q++) st<S>(a.gout, q * a.P + id, g0[q], a.gshift[q]);
@@ -1147,13 +1252,33 @@ __global__ void __launch_bounds__(256) k_fused_materialize(FArgs a, float* out) 
     for (int q = 0; q < L::Q; q++) out[q * a.N + p.id] = v[q];
 }
 
-// FP32 SoA -> storage (with shift)
+// FP32 SoA g -> storage (with the global shifts)
 template <typename S>
-__global__ void k_to_storage(const float* __restrict__ in, void* out, int64_t N, int Q, FArgs a, int g) {
+__global__ void k_to_storage(const float* __restrict__ in, void* out, int64_t N, int Q, FArgs a) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= N * Q) return;
     const int q = (int)(i / N);
-    st<S>(out, q * a.P + (i - q * N), in[i], g ? a.gshift[q] : a.fshift[q]);
+    st<S>(out, q * a.P + (i - q * N), in[i], a.gshift[q]);
+}
+
+// FP32 SoA f -> storage in a.fout; FP16 storage writes each node's reference (the sum of
+// its populations) to a.rout and the deviations from it
+template <class L, typename S>
+__global__ void __launch_bounds__(256) k_to_storage_f(const float* __restrict__ in, FArgs a) {
+    const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (id >= a.N) return;
+    float v[L::Q];
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) v[q] = in[q * a.N + id];
+    if constexpr (kNodeRef<S>) {
+        float rinv;
+        const float R = put_ref<S>(a, id, rho_sym<L, float>(v), rinv);
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) stf_ref<L>(a.fout, q * a.P + id, q, v[q], R, rinv);
+    } else {
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, v[q], a.fshift[q]);
+    }
 }
 
 struct Fused {
@@ -1182,6 +1307,9 @@ struct Fused {
     int64_t P = 0;                 // population plane pitch
     int face_table = 0;            // faces whose links differ from the periodic wrap
     float fshift[15] = {0}, gshift[15] = {0};
+    bool noderef = false;          // FP16N: f against per-node references
+    _Float16* ra = nullptr;        // FP16N: per-node references of fa and fb
+    _Float16* rb = nullptr;
     bool march = false;            // single-pass kernel (isothermal, fully periodic D3Q15)
     bool tmarch = false;           // thermal single-pass kernel (xuMRT + myMRT, fully periodic D3Q15)
     bool march_lds = false;        // the isothermal single pass with LDS-staged pulls (FP32)
@@ -1276,6 +1404,8 @@ FArgs make_args(mplb_case* c) {
     a.mper = z->mper;
     a.mzc = z->mzc;
     std::memcpy(a.fshift, z->fshift, sizeof(a.fshift));
+    a.rin = z->ra;
+    a.rout = z->rb;
     std::memcpy(a.gshift, z->gshift, sizeof(a.gshift));
     return a;
 }
@@ -1416,8 +1546,14 @@ int run_prologue(mplb_case* c) {
     for (int fld = 0; fld < (z->thermal ? 2 : 1); fld++) {
         FCHK(c, hipMemcpyAsync(tmp, c->f[fld], bytes, hipMemcpyDeviceToDevice, c->stream));
         GenericOps<float>::collide(c->mesh.lbmodel, c->stream, c->eos, c->fc[fld].mp, st, tmp, c->rho0);
-        void* dst = fld == 0 ? z->fa : z->ga;
-        k_to_storage<S><<<(unsigned)((N * Q + 255) / 256), 256, 0, c->stream>>>(tmp, dst, N, Q, a, fld);
+        if (fld == 0) {
+            // fa (with its references ra when FP16) is the input of the next step
+            a.fout = z->fa;
+            a.rout = z->ra;
+            k_to_storage_f<L, S><<<(unsigned)((N + 255) / 256), 256, 0, c->stream>>>(tmp, a);
+        } else {
+            k_to_storage<S><<<(unsigned)((N * Q + 255) / 256), 256, 0, c->stream>>>(tmp, z->ga, N, Q, a);
+        }
     }
     if (z->thermal) {
         FCHK(c, hipMemcpyAsync(z->T[z->cur], c->temp, sizeof(float) * N, hipMemcpyDeviceToDevice, c->stream));
@@ -1456,6 +1592,7 @@ int do_step(mplb_case* c) {
     if (c->timing) launch_step_timed<L, S>(c, a, z->thermal, false);
     else launch_step<L, S>(c, a, z->thermal, false);
     std::swap(z->fa, z->fb);
+    std::swap(z->ra, z->rb);
     std::swap(z->ga, z->gb);
     if (z->thermal) z->cur = 1 - z->cur;
     z->generic_current = false;
@@ -1472,6 +1609,7 @@ int materialize(mplb_case* c) {
     const dim3 grid((unsigned)a.nblocks), block(256);
     // the last step's inputs: post-collision of level n in fb/gb, old T/U = previous
     a.fin = z->fb;
+    a.rin = z->rb;
     a.gin = z->gb;
     a.draw = z->last_draw;
     if (z->thermal) {
@@ -1482,6 +1620,7 @@ int materialize(mplb_case* c) {
     if (z->thermal) k_fused_materialize<L, S, true><<<grid, block, 0, c->stream>>>(a, (float*)c->f[1]);
     // macroscopic outputs: re-run the collide pass (deterministic, rewrites fa)
     a.fout = z->fa;
+    a.rout = z->ra;
     a.gout = z->ga;
     a.first_call = 0;
     if (z->thermal) {
@@ -1517,9 +1656,11 @@ template <typename Fn>
 int dispatch(mplb_case* c, Fn fn) {
     Fused* z = F(c);
     if (z->lb == 0) {
+        if (z->noderef) return fn(D2Q9{}, h16n{});
         if (z->half) return fn(D2Q9{}, _Float16{});
         return fn(D2Q9{}, float{});
     }
+    if (z->noderef) return fn(D3Q15{}, h16n{});
     if (z->half) return fn(D3Q15{}, _Float16{});
     return fn(D3Q15{}, float{});
 }
@@ -1603,11 +1744,16 @@ int fused_create(mplb_case* c) {
     const int Q = m.Q;
     z->lb = m.lbmodel;
     z->thermal = c->fc[1].present && c->update[1];
-    z->half = c->opts.precision == MPLB_FP16S;
+    z->half = c->opts.precision == MPLB_FP16S || c->opts.precision == MPLB_FP16N;
+    z->noderef = c->opts.precision == MPLB_FP16N;
     const size_t sb = z->half ? 2 : 4;
     z->P = pitch_for(N);
     FCHK(c, hipMalloc(&z->fa, sb * Q * z->P));
     FCHK(c, hipMalloc(&z->fb, sb * Q * z->P));
+    if (z->noderef) {
+        FCHK(c, hipMalloc(&z->ra, sizeof(_Float16) * N));
+        FCHK(c, hipMalloc(&z->rb, sizeof(_Float16) * N));
+    }
     if (z->thermal) {
         FCHK(c, hipMalloc(&z->ga, sb * Q * z->P));
         FCHK(c, hipMalloc(&z->gb, sb * Q * z->P));
@@ -1702,12 +1848,16 @@ int fused_create(mplb_case* c) {
             // kernel's rho/T/U stencils of the heat source under the psi stencil)
             int bpc = 0, cus = 0;
             const int halo = z->tmarch ? 4 : 2;
-            if (z->tmarch && z->half)
+            if (z->tmarch && z->noderef)
+                FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_tmarch<D3Q15, h16n, false>, TNT, 0));
+            else if (z->tmarch && z->half)
                 FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_tmarch<D3Q15, _Float16, false>, TNT, 0));
             else if (z->tmarch)
                 FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_tmarch<D3Q15, float, false>, TNT, 0));
             else if (z->march_lds)
                 FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march_lds<D3Q15, false>, 256, 0));
+            else if (z->noderef)
+                FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march<D3Q15, h16n, false>, 256, 0));
             else if (z->half)
                 FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march<D3Q15, _Float16, false>, 256, 0));
             else
@@ -1752,7 +1902,7 @@ int fused_create(mplb_case* c) {
 void fused_destroy(mplb_case* c) {
     Fused* z = F(c);
     if (!z) return;
-    void* ptrs[] = {z->fa, z->fb, z->ga, z->gb, z->rho, z->sumg, z->T[0], z->T[1], z->U[0], z->U[1],
+    void* ptrs[] = {z->fa, z->fb, z->ra, z->rb, z->ga, z->gb, z->rho, z->sumg, z->T[0], z->T[1], z->U[0], z->U[1],
                     z->bcf, z->bcg, z->bcTw, z->bcPert, z->rnd_off};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);     // teardown: nothing left to report to
@@ -1763,8 +1913,9 @@ void fused_destroy(mplb_case* c) {
 static int set_shifts(mplb_case* c) {
     Fused* z = F(c);
     if (!z->half) return 0;
-    // deviation storage: f_k - w_k rho_ref, g_k - geq_k(T_ref, 0); reference values are
-    // the lattice means of the initial fields
+    // deviation storage: f_k - w_k rho_ref (FP16S; FP16N stores f against per-node
+    // references), g_k - geq_k(T_ref, 0); reference values are the lattice means of the
+    // initial fields
     const int64_t N = c->mesh.N;
     std::vector<float> h(N);
     FCHK(c, hipMemcpy(h.data(), c->rho, sizeof(float) * N, hipMemcpyDeviceToHost));

@@ -81,9 +81,12 @@ def test_batch_fp64_matches_oracle_bit_for_bit():
 
 # FP32 arithmetic: psi = sqrt(2 (p - rho cs2) / G) loses digits to cancellation where p
 # is close to rho cs2 (the vapour side), so after 300 steps the density differs from the
-# FP64 oracle by a few 1e-5; FP16 deviation storage keeps about 3 decimal digits of
-# f - w rho_ref.  Pressure adds the psi Laplacian, one more difference.
-@pytest.mark.parametrize("precision,tol", [(M.FP32, 1e-4), (M.FP16S, 5e-3)])
+# FP64 oracle by a few 1e-5; FP16 storage keeps about 3 decimal digits of f - w rho_ref
+# (FP16S, one reference density per case) or of each population's relative deviation from
+# w R, R its node's own density (FP16N: both phases keep those digits; the deepest
+# quench, T = 0.6 Tc, is the worst case).  Pressure adds the psi Laplacian, one more
+# difference.
+@pytest.mark.parametrize("precision,tol", [(M.FP32, 1e-4), (M.FP16S, 5e-3), (M.FP16N, 1e-3)])
 def test_batch_reduced_precision_tracks_oracle(precision, tol):
     steps = 300
     b, rhos, temps = make_batch(SMALL, precision)
@@ -176,7 +179,7 @@ def test_fp32_batch_mass_does_not_drift():
         assert abs(m / m0 - 1.0) < 1e-5
 
 
-@pytest.mark.parametrize("precision", [M.FP64, M.FP32, M.FP16S])
+@pytest.mark.parametrize("precision", [M.FP64, M.FP32, M.FP16S, M.FP16N])
 def test_single_pass_equals_two_pass(precision):
     """The single-pass kernel (rows of up to four equal-size cases per workgroup, psi in an
     LDS ring) does each node's arithmetic exactly as the two-pass kernels: identical

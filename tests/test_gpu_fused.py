@@ -38,17 +38,22 @@ def test_fused_fp32_tracks_oracle(name):
     assert du < 2e-2
 
 
+@pytest.mark.parametrize("precision,tol", [(M.FP16S, 2e-2), (M.FP16N, 2e-4)], ids=["fp16s", "fp16n"])
 @pytest.mark.parametrize("name", ["d3q15_bubble_nht", "d3q15_thermal_column", "d2q9_column_fixedT"])
-def test_fused_fp16_storage_error(name):
+def test_fused_fp16_storage_error(name, precision, tol):
+    """FP16S keeps f - w rho_ref with one reference density: the light phase's deviations
+    are of the order of the density jump, ~1e-2 of rho after the run.  FP16N keeps each
+    population relative to its own node's density, so both phases keep the FP16
+    mantissa: ~1e-5."""
     spec, fk, ht, frozen, steps = CASES[name]
     fields = sphere_fields(spec, seed=7, **fk)
-    c, o = pair(spec, fields, precision=M.FP16S, ht=ht, frozen=frozen, fused=True)
+    c, o = pair(spec, fields, precision=precision, ht=ht, frozen=frozen, fused=True)
     c.step(steps)
     o.run(steps)
     e = rel(c.get("rho"), o.get("rho"))
-    print("  fp16 storage rho err vs FP64 oracle %.2e" % e, flush=True)
+    print("  fp16 storage (precision %d) rho err vs FP64 oracle %.2e" % (precision, e), flush=True)
     assert np.isfinite(c.get("f")).all()
-    assert e < 2e-2
+    assert e < tol
 
 
 def test_fused_readback_does_not_perturb_the_run():
@@ -67,14 +72,16 @@ def test_fused_readback_does_not_perturb_the_run():
         assert np.array_equal(a.get(f), b.get(f)), f
 
 
-@pytest.mark.parametrize("precision", [M.FP32, M.FP16S])
+@pytest.mark.parametrize("precision", [M.FP32, M.FP16S, M.FP16N])
 def test_march_matches_two_pass(precision, monkeypatch):
     """The single-pass kernel follows the two-pass kernels on a periodic D3Q15 bubble, with
     FP32 and FP16 storage; psi is computed with the same arithmetic in both, so the
     trajectories agree to FP32
     contraction differences: 1e-5 relative on rho and f, and 1e-6 absolute on U (lattice
     units; a rounding-level difference moves U by ~1e-7, a wrong population by ~1e-3,
-    while max |U| is itself small, so a bound relative to it would be fragile)."""
+    while max |U| is itself small, so a bound relative to it would be fragile).  With FP16
+    storage a one-ulp FP32 difference can change a stored FP16 value, so the bars are ten
+    times wider."""
     spec, fk, ht, frozen, steps = CASES["d3q15_bubble_nht"]
     fields = sphere_fields(spec, seed=11, **fk)
     monkeypatch.setenv("MPLB_FUSED_KERNEL", "twopass")
@@ -83,13 +90,14 @@ def test_march_matches_two_pass(precision, monkeypatch):
     one, _ = pair(spec, fields, precision=precision, ht=False, fused=True)
     two.step(steps)
     one.step(steps)
+    widen = 1 if precision == M.FP32 else 10
     for f in ("rho", "f"):
         e = rel(one.get(f), two.get(f))
         print("  %s rel diff %.2e" % (f, e), flush=True)
-        assert e < 1e-5
+        assert e < 1e-5 * widen
     du = float(np.max(np.abs(one.get("U") - two.get("U"))))
     print("  U abs diff %.2e (max |U| %.2e)" % (du, float(np.max(np.abs(two.get("U"))))), flush=True)
-    assert du < 1e-6
+    assert du < 1e-6 * widen
 
 
 @pytest.mark.parametrize("kernel", ["single-pass", "twopass"])

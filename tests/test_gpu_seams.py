@@ -3,9 +3,10 @@
 * k_fused_march (the isothermal D3Q15 single pass) works on 64 x 4 columns of sites and
   z chunks: a 130 x 12 x 40 periodic bubble has three x tiles (the last one 2 sites wide),
   three column rows in y and five 8-plane z chunks, so ring sites cross tile and chunk
-  boundaries everywhere.  It must equal the two-pass fused kernels (same per-node
-  arithmetic) and track the FP64 oracle; FP16 storage likewise against its two passes,
-  and the FP32 variant that stages each plane's pulls in LDS likewise.
+  boundaries everywhere.  It must follow the two-pass fused kernels (same per-node
+  arithmetic; the compiler may fuse a multiply-add differently in the two kernels, so the
+  bar is FP32 rounding) and track the FP64 oracle; FP16 storage likewise against its two
+  passes, and the FP32 variant that stages each plane's pulls in LDS likewise.
 * k_fused_tmarch (the thermal D3Q15 single pass: xuMRT + myMRT with heat transfer)
   keeps rho, T and U on a two-site ring and psi on a one-site ring in LDS; on a
   130 x 10 x 24 periodic thermal bubble (three x tiles, a partial y tile, z chunks) it
@@ -35,8 +36,8 @@ SEAM = d3(nx=130, ny=12, nz=40, btype=5, g_model=None, write_interval=10 ** 6)
 SEAM_FIELDS = dict(T=0.8 * TC, r_in=0.03, r_out=0.16, radius=5.0, centre=(63.0, 5.5, 19.5))
 
 
-@pytest.mark.parametrize("precision,staged", [(M.FP32, False), (M.FP32, True), (M.FP16S, False)],
-                         ids=["fp32", "fp32-staged", "fp16s"])
+@pytest.mark.parametrize("precision,staged", [(M.FP32, False), (M.FP32, True), (M.FP16S, False), (M.FP16N, False)],
+                         ids=["fp32", "fp32-staged", "fp16s", "fp16n"])
 def test_march_across_tiles_and_chunks_equals_two_pass(precision, staged, monkeypatch):
     """staged: the variant that loads each plane's pulls straight into LDS
     (MPLB_MARCH_LDS=1); a read-back half way must leave its trajectory unchanged."""
@@ -57,11 +58,14 @@ def test_march_across_tiles_and_chunks_equals_two_pass(precision, staged, monkey
         assert one.step(steps - steps // 2) == steps - steps // 2
     else:
         assert one.step(steps) == steps
+    # rho and f relative to their largest value, U absolute (lattice units: a wrong
+    # population moves U by ~1e-3, a rounding difference by ~1e-7)
+    tol = {"rho": 1e-5, "f": 1e-5, "U": 1e-6} if precision == M.FP32 else {"rho": 1e-4, "f": 1e-4, "U": 1e-5}
     for f in ("rho", "U", "f"):
         a, b = one.get(f), two.get(f)
-        print("  %s single vs two pass: exact %s, max diff %.2e" % (f, np.array_equal(a, b), np.max(np.abs(a - b))),
-              flush=True)
-        assert np.array_equal(a, b), f
+        e = rel(a, b) if f != "U" else float(np.max(np.abs(a - b)))
+        print("  %s single vs two pass: exact %s, diff %.2e" % (f, np.array_equal(a, b), e), flush=True)
+        assert e < tol[f], f
     if precision == M.FP32:
         o.run(steps)
         errs = {f: rel(one.get(f), o.get(f)) for f in ("rho", "f")}
@@ -73,7 +77,7 @@ TSEAM = d3(nx=130, ny=10, nz=24, btype=5, write_interval=10 ** 6)
 TSEAM_FIELDS = dict(T=0.85 * TC, r_in=0.03, r_out=0.16, radius=4.5, centre=(64.0, 4.5, 11.5))
 
 
-@pytest.mark.parametrize("precision", [M.FP32, M.FP16S])
+@pytest.mark.parametrize("precision", [M.FP32, M.FP16S, M.FP16N])
 def test_thermal_march_tracks_three_passes(precision, monkeypatch):
     """Same per-node arithmetic as the three passes; the compiler may still fuse a
     multiply-add differently in the two kernels, so the comparison allows FP32 rounding
