@@ -452,6 +452,7 @@ def run_warmstart(args):
     else:
         train3 = bubble3d_cases(args.ws_3d_offset + args.ws_3d, 13, L3, 0.5)[args.ws_3d_offset:]
     kept3 = flat3 = moving3 = 0
+    plateau_rows = []          # (T/Tc, R, droplet, outside and inside plateaus) of steady 3D interfaces
     for i, c in enumerate(train3):
         start(cold_field(c))
         st, wall, ok = run_to_steady_case(case, c["rho_l"], thr, 100, args.max_steps)
@@ -470,6 +471,9 @@ def run_warmstart(args):
             kept3 += kept and ok
             moving3 += kept and not ok
             flat3 += not kept
+            if kept and ok:
+                _, Rs = W.extract_profile(rho, r3, c["rho_v"], c["rho_l"], c["droplet"])
+                plateau_rows.append((c["tr"], Rs, float(c["droplet"]), float(y[-1]), float(y[0])))
             samples_X.append(W.features(c["tr"], c["radius"], L3, c["droplet"], 3))
             samples_Y.append(np.concatenate([W.target_from_profile(y, c["droplet"]), np.zeros(W.NPROF)]))
         if i % 16 == 0:
@@ -478,6 +482,7 @@ def run_warmstart(args):
         time.perf_counter() - t3, kept3, moving3, flat3))
     X = np.array(samples_X, np.float32).reshape(-1, W.N_IN)
     Y = np.array(samples_Y, np.float32).reshape(-1, W.N_OUT)
+    P = np.array(plateau_rows, np.float64).reshape(-1, 5)
     loaded = {}
     if args.ws_load_data:
         # profiles from earlier solver runs of this workload (same generators, saved with
@@ -486,9 +491,11 @@ def run_warmstart(args):
             z = np.load(path)
             X = np.concatenate([X, z["X"]])
             Y = np.concatenate([Y, z["Y"]])
+            if "P" in z.files and z["P"].shape[1] == 5:
+                P = np.concatenate([P, z["P"]])
             loaded[os.path.basename(path)] = int(z["X"].shape[0])
     if args.ws_save_data:
-        np.savez(args.ws_save_data, X=X, Y=Y)
+        np.savez(args.ws_save_data, X=X, Y=Y, P=P)
     if args.ws_eval == 0:
         print(json.dumps({"saved_samples": int(X.shape[0]), "path": args.ws_save_data}))
         return 0
@@ -504,13 +511,24 @@ def run_warmstart(args):
     hist = W.train(net, X, Y, iters=args.ws_iters, seed=1)
     # second network: the interface profiles alone (the first one decides flat or not)
     keep = ~W.flat_targets(X, Y)
-    net_k = M.Mlp(W.WIDTHS, args.ws_batch)
-    net_k.set_params(W.init_params(W.WIDTHS, 1))
-    hist_k = W.train(net_k, X[keep], Y[keep], iters=args.ws_iters_kept, seed=2)
+    # (--ws-kept-nets of them from different seeds; the proposal is their mean)
+    net_k = []
+    for k in range(args.ws_kept_nets):
+        nk = M.Mlp(W.WIDTHS, args.ws_batch)
+        nk.set_params(W.init_params(W.WIDTHS, 1 + 10 * k))
+        hist_k = W.train(nk, X[keep], Y[keep], iters=args.ws_iters_kept, seed=2 + 10 * k)
+        net_k.append(nk)
     t_train = time.perf_counter() - t_train
-    log("trained %d samples (%d interface profiles), 2 x %d params, %d + %d iters in %.1f s, loss %.3e -> %.3e "
-        "(interfaces: %.3e -> %.3e)" % (X.shape[0], int(keep.sum()), net.P, args.ws_iters, args.ws_iters_kept, t_train,
-                                        hist[:20].mean(), hist[-50:].mean(), hist_k[:20].mean(), hist_k[-50:].mean()))
+    log("trained %d samples (%d interface profiles), 1 + %d x %d params, %d + %d iters in %.1f s, loss %.3e -> %.3e "
+        "(interfaces, last network: %.3e -> %.3e)" % (
+            X.shape[0], int(keep.sum()), args.ws_kept_nets, net.P, args.ws_iters, args.ws_iters_kept, t_train,
+            hist[:20].mean(), hist[-50:].mean(), hist_k[:20].mean(), hist_k[-50:].mean()))
+    # the outside plateau from the steady 3D interfaces (a second model; see warmstart.py)
+    pfit, prms = W.fit_plateau(P[:, 0], P[:, 1], P[:, 2] > 0, P[:, 3]) if len(P) else ({}, {})
+    pfit_in, prms_in = W.fit_plateau(P[:, 0], P[:, 1], P[:, 2] > 0, P[:, 4]) if len(P) else ({}, {})
+    log("plateau fits: %d steady 3D interfaces (%d droplets), rms residual outside %s / %s, inside %s / %s "
+        "(droplets / bubbles, of the jump)" % (len(P), int((P[:, 2] > 0).sum()) if len(P) else 0, prms.get(True),
+                                              prms.get(False), prms_in.get(True), prms_in.get(False)))
     evals = bubble3d_cases(args.ws_eval, 17, L3, args.ws_eval_droplets)
     inner, outer = r3 < 2.0, r3 > r3.max() - 2.0
     rows = []
@@ -520,45 +538,83 @@ def run_warmstart(args):
         start(rho0)
         st_c, wall_c, ok_c = run_to_steady_case(case, c["rho_l"], thr, 100, args.max_steps)
         rc = case.get("rho")
-        # warm: network proposal -> mass-matched raster -> the solver to its own criterion;
-        # a proposal outside the physical range, or a warm run that diverges, falls back
-        # to the cold start, whose time is then added to the warm total
+        finite_c = bool(np.isfinite(rc).all())
+        cin, cout = rc[inner].mean(), rc[outer].mean()
+        kept = finite_c and abs(cin - cout) >= 0.25 * (c["rho_l"] - c["rho_v"])
+        # warm: proposal -> mass-matched raster -> the solver to its own criterion; a
+        # proposal outside the physical range, or a warm run that diverges, falls back to
+        # the cold start, whose time is then added to the warm total.  Two rasters of the
+        # same proposal: with the fitted outside plateau (the method) and the network's own
+        # (for comparison); a uniform proposal is the same for both.
         tw = time.perf_counter()
         prof, flat = W.propose(net, net_k, W.features(c["tr"], c["radius"], L3, c["droplet"], 3), c["droplet"],
                                c["rho_v"], c["rho_l"])
         used = W.proposal_ok(prof, c["rho_v"], c["rho_l"])
         flat = used and flat
-        Rw = W.raster_case(case, prof, cen, mass, c["radius"], int(sites3.sum()), uniform=flat) if used else None
-        if used:
-            start(None)
-            used = bool(np.isfinite(case.get("rho")).all())
         t_prop = time.perf_counter() - tw
-        fallback = not used
-        st_w = wall_w = 0.0
-        ok_w = False
-        if used:
-            st_w, wall_w, ok_w = run_to_steady_case(case, c["rho_l"], thr, 100, args.max_steps)
-            fallback = not ok_w and not np.isfinite(case.get("rho")).all()
-        if fallback:
-            start(rho0)
-            s2, w2, ok_w = run_to_steady_case(case, c["rho_l"], thr, 100, args.max_steps)
-            st_w, wall_w = st_w + s2, wall_w + w2
-        wall_w += t_prop
-        rw = case.get("rho")
-        finite_c = bool(np.isfinite(rc).all())
-        cin, cout = rc[inner].mean(), rc[outer].mean()
-        win, wout = rw[inner].mean(), rw[outer].mean()
-        dev = float(max(abs(win - cin) / cin, abs(wout - cout) / cout)) if finite_c else float("nan")
-        kept = finite_c and abs(cin - cout) >= 0.25 * (c["rho_l"] - c["rho_v"])
-        rows.append(dict(steps_cold=st_c, steps_warm=st_w, wall_cold=wall_c, wall_warm=wall_w, ok_cold=ok_c,
-                         ok_warm=ok_w, finite_cold=finite_c, kept=kept, droplet=c["droplet"], dev=dev,
-                         fallback=fallback, flat=flat, prop_s=t_prop))
-        log("eval %d (%s R %.1f T/Tc %.3f): cold %d steps %.2f s%s, warm %d steps %.2f s%s%s (proposal %.3f s, %s), "
-            "cold in/out %.5f %.5f%s, dev %.1e" % (
+        res = {}
+        for variant in ("plateau", "net"):
+            if variant == "net" and (flat or not used):
+                res["net"] = res["plateau"]
+                continue
+            # timed: the proposal and its raster (the set-up of U, T and the first collision
+            # that follows is the cold run's set-up too, untimed in both)
+            tr0 = time.perf_counter()
+            Rw = None
+            ok_raster = used
+            if used:
+                if variant == "plateau" and not flat:
+                    y = (np.asarray(prof, np.float64) - c["rho_v"]) / (c["rho_l"] - c["rho_v"])
+                    Rw = W.raster_case_plateau(case, y, pfit, c["tr"], c["droplet"], c["rho_v"], c["rho_l"], cen, mass,
+                                               c["radius"], int(sites3.sum()), fit_in=pfit_in)
+                else:
+                    Rw = W.raster_case(case, prof, cen, mass, c["radius"], int(sites3.sum()), uniform=flat)
+            t_r = time.perf_counter() - tr0
+            if used:
+                start(None)
+                ok_raster = bool(np.isfinite(case.get("rho")).all())
+            fallback = not ok_raster
+            st_w = wall_w = 0.0
+            ok_w = False
+            if ok_raster:
+                st_w, wall_w, ok_w = run_to_steady_case(case, c["rho_l"], thr, 100, args.max_steps)
+                fallback = not ok_w and not np.isfinite(case.get("rho")).all()
+            if fallback:
+                start(rho0)
+                s2, w2, ok_w = run_to_steady_case(case, c["rho_l"], thr, 100, args.max_steps)
+                st_w, wall_w = st_w + s2, wall_w + w2
+            wall_w += t_prop + t_r
+            rw = case.get("rho")
+            win, wout = rw[inner].mean(), rw[outer].mean()
+            dev = float(max(abs(win - cin) / cin, abs(wout - cout) / cout)) if finite_c else float("nan")
+            res[variant] = dict(steps=st_w, wall=wall_w, ok=ok_w, fallback=fallback, dev=dev, R=Rw)
+        w, wn = res["plateau"], res["net"]
+        perr = None
+        if kept and used and not flat and ok_c:
+            # the proposal against the cold run's steady radial profile (normalised): inside
+            # plateau, interface (|d| <= 6) and outside plateau, and the interface radius
+            yt, Rc = W.extract_profile(rc, r3, c["rho_v"], c["rho_l"], c["droplet"])
+            yp = (np.asarray(prof, np.float64) - c["rho_v"]) / (c["rho_l"] - c["rho_v"])
+            ypl = W.plateau_profile(yp, pfit, pfit_in, c["tr"], w["R"] if w["R"] else c["radius"], c["droplet"])
+            band = np.abs(W.DK) <= 6.0
+            perr = {"in": float(ypl[0] - yt[0]), "out": float(ypl[-1] - yt[-1]),
+                    "out_net": float(yp[-1] - yt[-1]), "in_net": float(yp[0] - yt[0]),
+                    "shape_rms": float(np.sqrt(np.mean((ypl[band] - yt[band]) ** 2))),
+                    "R": float((w["R"] or 0.0) - Rc)}
+        rows.append(dict(steps_cold=st_c, steps_warm=w["steps"], wall_cold=wall_c, wall_warm=w["wall"], ok_cold=ok_c,
+                         ok_warm=w["ok"], finite_cold=finite_c, kept=kept, droplet=c["droplet"], dev=w["dev"],
+                         fallback=w["fallback"], flat=flat, prop_s=t_prop, steps_net=wn["steps"], wall_net=wn["wall"],
+                         ok_net=wn["ok"], dev_net=wn["dev"], perr=perr))
+        log("eval %d (%s R %.1f T/Tc %.3f): cold %d steps %.2f s%s, warm %d steps %.2f s%s%s (%s), network-only "
+            "plateau %d steps, cold in/out %.5f %.5f%s, dev %.1e / %.1e" % (
                 i, "drop" if c["droplet"] else "bubl", c["radius"], c["tr"], st_c, wall_c, "" if ok_c else " (not steady)",
-                st_w, wall_w, "" if ok_w else " (not steady)", " (fell back to cold)" if fallback else "", t_prop,
-                "uniform" if flat else ("R %.2f" % Rw if Rw is not None else "not used"),
-                cin, cout, "" if kept else " (interface gone)", dev))
+                w["steps"], w["wall"], "" if w["ok"] else " (not steady)", " (fell back to cold)" if w["fallback"] else "",
+                "uniform" if flat else ("R %.2f" % w["R"] if w["R"] is not None else "not used"), wn["steps"],
+                cin, cout, "" if kept else " (interface gone)", w["dev"], wn["dev"]))
+        if perr:
+            log("   proposal vs steady profile: inside %+.1e (network %+.1e), outside %+.1e (network %+.1e), "
+                "interface rms %.1e, radius %+.3f" % (perr["in"], perr["in_net"], perr["out"], perr["out_net"],
+                                                      perr["shape_rms"], perr["R"]))
     # population: held-out cases whose cold run stays finite (the plain solver's result
     # exists); a cold run still moving at the step limit counts with the limit (a lower
     # bound of its time).  Agreement is judged where both runs met the steady criterion.
@@ -586,8 +642,9 @@ def run_warmstart(args):
             "bubbles" if args.ws_eval_droplets == 0 else "droplets/bubbles", L3, L3 // 4),
                    "steady": "max|drho|/rho_l over 100 steps < %g" % thr, "max_steps": args.max_steps,
                    "train_2d": args.ws_2d, "train_3d": args.ws_3d, "loaded_samples": loaded, "eval": len(evals),
-                   "network": {"widths": W.WIDTHS, "params": net.P, "networks": 2,
-                               "roles": "all steady states (dissolves or not); interface profiles",
+                   "network": {"widths": W.WIDTHS, "params": net.P, "networks": 1 + args.ws_kept_nets,
+                               "roles": "all steady states (dissolves or not); interface profiles (mean of %d)" %
+                                        args.ws_kept_nets,
                                "iters": [args.ws_iters, args.ws_iters_kept], "batch": args.ws_batch,
                                "3d_repeat": args.ws_3d_repeat}},
         "cases": {"eval": len(rows), "cold_unstable": len(rows) - len(pop), "compared": len(pop),
@@ -600,6 +657,18 @@ def run_warmstart(args):
         "speedup_wall": {k: ratio(g, "wall") for k, g in groups.items()},
         "speedup_steps": {"all": ratio(pop, "steps"), **{k: ratio(g, "steps") for k, g in groups.items()}},
         "speedup_steps_per_case": per_case(pop, "steps"),
+        "network_plateau_only": {
+            "speedup_wall": {k: round(float(np.sum([r["wall_cold"] for r in g]) / np.sum([r["wall_net"] for r in g])), 3)
+                             if g else None for k, g in groups.items()},
+            "speedup_steps": {k: round(float(np.sum([r["steps_cold"] for r in g]) / np.sum([r["steps_net"] for r in g])), 3)
+                              if g else None for k, g in groups.items()},
+            "dev_max": float(max([r["dev_net"] for r in pop if r["ok_cold"] and r["ok_net"]], default=0.0))},
+        "proposal_error_kept": {k: round(float(np.median(np.abs([r["perr"][k] for r in pop if r["perr"]]))), 8)
+                                for k in ("in", "in_net", "out", "out_net", "shape_rms", "R")}
+        if any(r["perr"] for r in pop) else None,
+        "plateau_fit": {"samples": int(len(P)), "rms_outside_droplets": prms.get(True),
+                        "rms_outside_bubbles": prms.get(False), "rms_inside_droplets": prms_in.get(True),
+                        "rms_inside_bubbles": prms_in.get(False)},
         "final_state_rel_dev": {"median": float(np.median(devs)), "max": float(devs.max()),
                                 "failures_over_1e-4": int((devs > 1e-4).sum()),
                                 **{k + "_max": float(max([r["dev"] for r in g if r in both], default=0.0))
@@ -759,6 +828,7 @@ def main():
     ap.add_argument("--ws-eval-droplets", type=float, default=0.0, help="warmstart: share of droplets among the eval cases")
     ap.add_argument("--ws-iters", type=int, default=4000, help="warmstart: Adam iterations of the all-cases network")
     ap.add_argument("--ws-iters-kept", type=int, default=16000, help="warmstart: iterations of the interface network")
+    ap.add_argument("--ws-kept-nets", type=int, default=3, help="warmstart: interface networks (the proposal is their mean)")
     ap.add_argument("--ws-batch", type=int, default=256)
     ap.add_argument("--ws-3d-repeat", type=int, default=4, help="warmstart: times each 3D profile enters the training set")
     ap.add_argument("--ws-3d-offset", type=int, default=0, help="warmstart: first 3D training case index")

@@ -118,3 +118,41 @@ def test_flat_targets_and_two_network_proposal():
     y_other = 0.5 * (1.0 + np.tanh(2.0 * W.DK / 6.0))
     prof, flat = W.propose(_Fixed(out(y_step)), _Fixed(out(y_other)), x_b, False, RV, RL)
     assert not flat and np.allclose(prof, W.profile_density(y_other, RV, RL), atol=1e-6)
+
+
+def test_plateau_fit_recovers_a_smooth_law_and_needs_enough_samples():
+    """The plateau model is least squares on s^i and s^i / R (s = sqrt(1 - T/Tc)): a law
+    in that span comes back exactly (to rounding), a flag with too few samples gets no
+    fit (the network's own plateau is then kept)."""
+    rng = np.random.default_rng(5)
+    n = 60
+    tr = 0.6 + 0.35 * rng.random(n)
+    R = 15.0 + 15.0 * rng.random(n)
+    s = np.sqrt(1.0 - tr)
+    law = 1.0 - 0.08 * s ** 2 + 0.01 * s ** 3 + 0.02 * s / R
+    droplet = np.zeros(n, bool)
+    droplet[:10] = True                      # fewer than PLATEAU_MIN_SAMPLES droplets
+    fit, rms = W.fit_plateau(tr, R, droplet, law)
+    assert fit[True] is None and rms[True] is None
+    assert rms[False] < 1e-12
+    for t, rr in ((0.65, 17.0), (0.8, 25.0), (0.93, 29.0)):
+        ss = np.sqrt(1.0 - t)
+        assert abs(W.plateau_at(fit, t, rr, False) - (1.0 - 0.08 * ss ** 2 + 0.01 * ss ** 3 + 0.02 * ss / rr)) < 1e-10
+    assert W.plateau_at(fit, 0.7, 20.0, True) is None
+
+
+def test_plateau_profile_moves_only_the_plateaus():
+    """plateau_profile puts the fitted values on the profile's ends (the smooth windows
+    reach 1 - 2e-6 there) and leaves the interface band (|d| < 3, window weight <= 0.02)
+    as proposed up to 2% of the plateau change; without fits the profile is unchanged."""
+    rng = np.random.default_rng(6)
+    n = 40
+    tr, R = 0.6 + 0.3 * rng.random(n), 16.0 + 14.0 * rng.random(n)
+    out_fit, _ = W.fit_plateau(tr, R, np.zeros(n, bool), np.full(n, 0.99))
+    in_fit, _ = W.fit_plateau(tr, R, np.zeros(n, bool), np.full(n, -0.01))
+    y = W.cold_profile(False)
+    yp = W.plateau_profile(y, out_fit, in_fit, 0.75, 22.0, False)
+    assert abs(yp[-1] - 0.99) < 1e-7 and abs(yp[0] - (-0.01)) < 1e-7
+    band = np.abs(W.DK) < 3.0
+    assert np.max(np.abs(yp[band] - y[band])) < 0.02 * 0.01
+    assert np.array_equal(W.plateau_profile(y, {}, {}, 0.75, 22.0, False), y)
