@@ -883,23 +883,29 @@ def copy_ceiling(achieved_gbs, size):
     """A flat copy of the 15 population planes of an size^3 lattice (read + write counted;
     multiphaselb_amd/stream_probe, run as a child process so that its HIP context is its
     own): the streaming rate the same box reaches without any stencil, next to the
-    dominant kernel's algorithmic rate."""
+    dominant kernel's algorithmic rate.  The probe times two copies (grid-stride over 8
+    workgroups per CU; one float4 per work item over the whole grid); the faster one is
+    the ceiling."""
     exe = os.path.join(ROOT, "multiphaselb_amd", "stream_probe")
     if not os.path.exists(exe):
         return None
     r = subprocess.run([exe, str(size), "copy"], capture_output=True, text=True, timeout=120)
     rows = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+    rows = [w for w in rows if w["pattern"].startswith("copy")]
     if r.returncode != 0 or not rows:
         return None
-    gbs = rows[0]["GBs"]
-    return {"GBs": gbs, "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4), "bytes": 120 * size ** 3,
-            "kernel_over_copy": round(achieved_gbs / gbs, 4)}
+    best = max(rows, key=lambda w: w["GBs"])
+    gbs = best["GBs"]
+    return {"GBs": gbs, "pattern": best["pattern"], "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4),
+            "bytes": 120 * size ** 3, "kernel_over_copy": round(achieved_gbs / gbs, 4),
+            "copies": {w["pattern"]: w["GBs"] for w in rows}}
 
 
 def pattern_probe(args):
     """Runs the memory-pattern probe (multiphaselb_amd/stream_probe, built with the library)
-    on an --size^3 lattice: a flat copy of the 15 population planes, the single pass's pull
-    pattern without and with the ring re-pulls, at 1-3 z chunks; no collision arithmetic."""
+    on an --size^3 lattice: two flat copies of the 15 population planes, the single pass's
+    pull pattern without and with the ring re-pulls, at 1-3 z chunks; no collision
+    arithmetic."""
     exe = os.path.join(ROOT, "multiphaselb_amd", "stream_probe")
     if not os.path.exists(exe):
         print("stream_probe is not built; run `python -m multiphaselb_amd.build`", file=sys.stderr)
@@ -945,9 +951,11 @@ def measure(args, thermal):
     # runs of this workload (profiles/fused_traffic.json), when one exists for it
     traffic = None
     staged = os.environ.get("MPLB_MARCH_LDS") == "1" and args.precision == "fp32"
-    march_name = "k_fused_march_lds" if staged else "k_fused_march"
+    ahead = (not staged and os.environ.get("MPLB_MARCH_PF") == "1" and args.precision in ("fp32", "fp16s"))
+    march_name = "k_fused_march_lds" if staged else ("k_fused_march_pf" if ahead else "k_fused_march")
+    variant = "_staged" if staged else ("_ahead" if ahead else "")
     key = "d3q15_%d_%s_%s" % (args.size, args.precision,
-                              "thermal" if thermal else ("iso" + ("_staged" if staged else "") if single else "iso_twopass"))
+                              "thermal" if thermal else ("iso" + variant if single else "iso_twopass"))
     tpath = os.path.join(ROOT, "profiles", "fused_traffic.json")
     if kname == "fused" and os.path.exists(tpath):
         rec = json.load(open(tpath)).get("kernels", {}).get(key)

@@ -724,6 +724,12 @@ static_assert(MX * MY == 256 && MRING <= 256, "one thread per column site and pe
 
 __device__ inline int wrapd(int u, int n) { return u < 0 ? u + (n - 1) : (u > n - 1 ? u - (n - 1) : u); }
 
+// s_waitcnt immediates (vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8): every
+// counter at zero; only the LDS counter at zero; only the vector-memory counter at zero
+constexpr int kWaitAll = 0;
+constexpr int kWaitLds = 0xF | 0x7 << 4 | 3 << 14;
+constexpr int kWaitVm = 0x7 << 4 | 0xF << 8;
+
 // populations of node (x, y, z) pulled from x - c_q (duplicate-node periodic wrap) and
 // their sum (REF: f stored against per-node references, FP16N)
 template <class L, typename S, bool REF>
@@ -848,6 +854,147 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPLB_M
         }
 #pragma unroll
         for (int q = 0; q < L::Q; q++) fc[q] = fn[q];
+    }
+}
+
+// The same single pass with its pulls one plane ahead (software-pipelined, FP32 and
+// FP16S storage): the loads of plane z + 2 (column and ring sites, plus T when it is not
+// uniform) are issued before plane z collides and land in registers as stored values, so
+// their latency runs under the collision of plane z; psi of plane z + 2 is formed before
+// plane z is stored.  The arithmetic per node is that of k_fused_march.  Two waves per
+// SIMD (the in-flight pulls hold 30 more registers).
+#ifndef MPLB_MARCH_PF_WAVES
+#define MPLB_MARCH_PF_WAVES 2
+#endif
+
+template <class L, typename S, bool OUT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPLB_MARCH_PF_WAVES)))
+k_fused_march_pf(FArgs a) {
+    static_assert(!kNodeRef<S>, "FP32 or FP16S storage");
+    using R = typename std::conditional<sizeof(S) == 2, _Float16, float>::type;
+    __shared__ float ring[4][MSLOT];
+    const unsigned wi = (blockIdx.x & 7u) * (unsigned)a.mper + (blockIdx.x >> 3);
+    if (wi >= (unsigned)a.mitems) return;
+    const int chunk = (int)(wi / (unsigned)a.mtiles), tile = (int)(wi - (unsigned)chunk * (unsigned)a.mtiles);
+    const int x0 = (tile % a.mtx) * MX, y0 = (tile / a.mtx) * MY;
+    const int z0 = chunk * a.mzc, z1 = min(z0 + a.mzc, a.nz);
+    const int t = threadIdx.x, tx = t % MX, ty = t / MX;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool owner = x < a.nx && y < a.ny;
+    const bool edge = x <= a.nx && y <= a.ny;          // raw n wraps onto site 1
+    const int ex = wrapd(x, a.nx), ey = wrapd(y, a.ny);
+    int rhx = 0, rhy = 0;
+    if (t < MHX) {
+        rhx = t;
+    } else if (t < 2 * MHX) {
+        rhx = t - MHX;
+        rhy = MHY - 1;
+    } else if (t < 2 * MHX + MY) {
+        rhy = 1 + (t - 2 * MHX);
+    } else {
+        rhx = MHX - 1;
+        rhy = 1 + (t - 2 * MHX - MY);
+    }
+    const bool ring_site = t < MRING && x0 + rhx - 1 <= a.nx && y0 + rhy - 1 <= a.ny;
+    const int rx = wrapd(x0 + rhx - 1, a.nx), ry = wrapd(y0 + rhy - 1, a.ny);
+    const unsigned nx = (unsigned)a.nx, nxy = (unsigned)a.nxy;
+    const unsigned EX[3] = {(unsigned)wrapd(ex - 1, a.nx), (unsigned)ex, (unsigned)wrapd(ex + 1, a.nx)};
+    const unsigned EY[3] = {(unsigned)wrapd(ey - 1, a.ny) * nx, (unsigned)ey * nx, (unsigned)wrapd(ey + 1, a.ny) * nx};
+    const unsigned RX[3] = {(unsigned)wrapd(rx - 1, a.nx), (unsigned)rx, (unsigned)wrapd(rx + 1, a.nx)};
+    const unsigned RY[3] = {(unsigned)wrapd(ry - 1, a.ny) * nx, (unsigned)ry * nx, (unsigned)wrapd(ry + 1, a.ny) * nx};
+    float fc[L::Q], fn[L::Q];
+    R pe[L::Q], pr[L::Q];              // pulls in flight: column site, ring site
+    float te = a.Tconst, tr = a.Tconst;
+    // issues the pulls of raw plane zr (stored values; nothing waits on them here)
+    auto issue = [&](int zr) {
+        const int zw = wrapd(zr, a.nz);
+        const unsigned Z[3] = {(unsigned)wrapd(zw - 1, a.nz) * nxy, (unsigned)zw * nxy, (unsigned)wrapd(zw + 1, a.nz) * nxy};
+        __builtin_amdgcn_s_setprio(MPLB_PULL_PRIO);
+        if (edge) {
+#pragma unroll
+            for (int q = 0; q < L::Q; q++)
+                pe[q] = ((const R*)a.fin)[(size_t)q * a.P +
+                                          (EX[1 - L::t.c[q][0]] + EY[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]])];
+            if (!a.Tuniform) te = a.Told[(int64_t)ex + (int64_t)ey * a.nx + (int64_t)zw * a.nxy];
+        }
+        if (ring_site) {
+#pragma unroll
+            for (int q = 0; q < L::Q; q++)
+                pr[q] = ((const R*)a.fin)[(size_t)q * a.P +
+                                          (RX[1 - L::t.c[q][0]] + RY[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]])];
+            if (!a.Tuniform) tr = a.Told[(int64_t)rx + (int64_t)ry * a.nx + (int64_t)zw * a.nxy];
+        }
+        __builtin_amdgcn_s_setprio(0);
+    };
+    auto stored = [&](R v, int q) -> float {
+        if constexpr (sizeof(S) == 2) return (float)v + a.fshift[q];
+        else return v;
+    };
+    // psi of raw plane zr from the landed pulls into its ring slot; column populations -> fo.
+    // It ends with an explicit wait for every outstanding memory operation (the pulls, and
+    // the previous plane's stores, issued before this plane's collision): the compiler
+    // cannot tell that a lane which skipped the pulls also skipped their use, and would
+    // otherwise make the next plane's address arithmetic wait on the stores issued in
+    // between (one counter for loads and stores, which complete in order).
+    auto finish = [&](int zr, float* fo) {
+        float* sl = ring[(zr - z0 + 1) & 3];
+        float v[L::Q];
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) {
+            fo[q] = stored(pe[q], q);
+            v[q] = stored(pr[q], q);
+        }
+        const float rho_e = rho_sym<L, float>(fo), rho_r = rho_sym<L, float>(v);
+        if (edge) sl[(ty + 1) * MHX + tx + 1] = fpot(a, rho_e, te);
+        if (ring_site) sl[rhy * MHX + rhx] = fpot(a, rho_r, tr);
+        __builtin_amdgcn_s_waitcnt(kWaitVm);
+    };
+    issue(z0 - 1);
+    finish(z0 - 1, fn);
+    issue(z0);
+    finish(z0, fc);
+    issue(z0 + 1);
+    finish(z0 + 1, fn);
+    __syncthreads();
+    for (int z = z0; z < z1; z++) {
+        // ring slots: z - 1, z, z + 1 are read by this collision; z + 2 takes the slot of
+        // z - 2, which every wave finished reading before the barrier that ended plane z - 1.
+        // psi of z + 2 is formed before plane z is stored, so that waiting for its pulls
+        // does not wait for those stores as well (one counter covers loads and stores).
+        const bool more = z + 2 <= z1;
+        if (more) issue(z + 2);
+        float fnn[L::Q];
+        float rho = 0.f, U[3], Fi[3], Ft[3];
+        if (owner) {
+            rho = rho_sym<L, float>(fc);
+            float psin[L::Q];
+#pragma unroll
+            for (int q = 0; q < L::Q; q++)
+                psin[q] = ring[(z + L::t.c[q][2] - z0 + 1) & 3][(ty + 1 + L::t.c[q][1]) * MHX + tx + 1 + L::t.c[q][0]];
+            collide_node<L>(a, fc, rho, psin, true, true, U, Fi, Ft);
+        }
+        if (more) finish(z + 2, fnn);
+        if (owner) {
+            const int64_t id = (int64_t)x + (int64_t)y * a.nx + (int64_t)z * a.nxy;
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, fc[q], a.fshift[q]);
+            if (OUT) {
+                const int64_t N = a.N;
+                a.out_rho[id] = rho;
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    a.out_U[d * N + id] = U[d];
+                    a.out_Fi[d * N + id] = Fi[d];
+                    a.out_Ft[d * N + id] = Ft[d];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) {
+            fc[q] = fn[q];
+            fn[q] = fnn[q];
+        }
+        __syncthreads();
     }
 }
 
@@ -1082,10 +1229,6 @@ constexpr int LST = MX * MY + MRING;       // staged sites: the column, then its
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// s_waitcnt immediates: every counter at zero; only the LDS counter at zero
-constexpr int kWaitAll = 0;
-constexpr int kWaitLds = 0xF | 0x7 << 4 | 3 << 14;
-
 // the workgroup meets; the memory clobber keeps the compiler from moving LDS accesses
 // across it
 __device__ inline void wg_barrier() { asm volatile("s_barrier" ::: "memory"); }
@@ -1313,6 +1456,7 @@ struct Fused {
     bool march = false;            // single-pass kernel (isothermal, fully periodic D3Q15)
     bool tmarch = false;           // thermal single-pass kernel (xuMRT + myMRT, fully periodic D3Q15)
     bool march_lds = false;        // the isothermal single pass with LDS-staged pulls (FP32)
+    bool march_pf = false;         // the isothermal single pass with its pulls a plane ahead
     bool t_uniform = false;        // the isothermal temperature field is one value (t_const)
     float t_const = 0.f;
     int mtx = 0, mtiles = 0, mitems = 0, mper = 0, mzc = 0;
@@ -1418,6 +1562,12 @@ void launch_march(mplb_case* c, const FArgs& a) {
         if constexpr (std::is_same<S, float>::value) {
             if (F(c)->march_lds) {
                 k_fused_march_lds<L, OUT><<<grid, 256, 0, c->stream>>>(a);
+                return;
+            }
+        }
+        if constexpr (!kNodeRef<S>) {
+            if (F(c)->march_pf) {
+                k_fused_march_pf<L, S, OUT><<<grid, 256, 0, c->stream>>>(a);
                 return;
             }
         }
@@ -1837,6 +1987,9 @@ int fused_create(mplb_case* c) {
         // LDS-staged pulls for the isothermal single pass, FP32 storage (opt-in, MPLB_MARCH_LDS=1)
         const char* lenv = std::getenv("MPLB_MARCH_LDS");
         z->march_lds = z->march && !z->half && lenv && std::strcmp(lenv, "1") == 0;
+        // pulls issued a plane ahead (FP32 and FP16S storage; opt-in, MPLB_MARCH_PF=1)
+        const char* penv = std::getenv("MPLB_MARCH_PF");
+        z->march_pf = z->march && !z->march_lds && !z->noderef && penv && std::strcmp(penv, "1") == 0;
         if (z->march || z->tmarch) {
             const int tmx = z->tmarch ? TMX : MX, tmy = z->tmarch ? TMY : MY;
             z->mtx = (m.nx + tmx - 1) / tmx;
@@ -1856,6 +2009,10 @@ int fused_create(mplb_case* c) {
                 FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_tmarch<D3Q15, float, false>, TNT, 0));
             else if (z->march_lds)
                 FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march_lds<D3Q15, false>, 256, 0));
+            else if (z->march_pf && z->half)
+                FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march_pf<D3Q15, _Float16, false>, 256, 0));
+            else if (z->march_pf)
+                FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march_pf<D3Q15, float, false>, 256, 0));
             else if (z->noderef)
                 FCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_fused_march<D3Q15, h16n, false>, 256, 0));
             else if (z->half)

@@ -6,7 +6,8 @@
   boundaries everywhere.  It must follow the two-pass fused kernels (same per-node
   arithmetic; the compiler may fuse a multiply-add differently in the two kernels, so the
   bar is FP32 rounding) and track the FP64 oracle; FP16 storage likewise against its two
-  passes, and the FP32 variant that stages each plane's pulls in LDS likewise.
+  passes, and the variants that stage each plane's pulls in LDS (FP32) or issue them a
+  plane early (FP32, FP16S) likewise.
 * k_fused_tmarch (the thermal D3Q15 single pass: xuMRT + myMRT with heat transfer)
   keeps rho, T and U on a two-site ring and psi on a one-site ring in LDS; on a
   130 x 10 x 24 periodic thermal bubble (three x tiles, a partial y tile, z chunks) it
@@ -36,23 +37,29 @@ SEAM = d3(nx=130, ny=12, nz=40, btype=5, g_model=None, write_interval=10 ** 6)
 SEAM_FIELDS = dict(T=0.8 * TC, r_in=0.03, r_out=0.16, radius=5.0, centre=(63.0, 5.5, 19.5))
 
 
-@pytest.mark.parametrize("precision,staged", [(M.FP32, False), (M.FP32, True), (M.FP16S, False), (M.FP16N, False)],
-                         ids=["fp32", "fp32-staged", "fp16s", "fp16n"])
-def test_march_across_tiles_and_chunks_equals_two_pass(precision, staged, monkeypatch):
+VARIANT_ENV = {"staged": "MPLB_MARCH_LDS", "ahead": "MPLB_MARCH_PF"}
+
+
+@pytest.mark.parametrize("precision,variant", [(M.FP32, ""), (M.FP32, "staged"), (M.FP32, "ahead"), (M.FP16S, ""),
+                                               (M.FP16S, "ahead"), (M.FP16N, "")],
+                         ids=["fp32", "fp32-staged", "fp32-ahead", "fp16s", "fp16s-ahead", "fp16n"])
+def test_march_across_tiles_and_chunks_equals_two_pass(precision, variant, monkeypatch):
     """staged: the variant that loads each plane's pulls straight into LDS
-    (MPLB_MARCH_LDS=1); a read-back half way must leave its trajectory unchanged."""
+    (MPLB_MARCH_LDS=1); ahead: the variant that issues each plane's pulls one plane
+    early (MPLB_MARCH_PF=1).  For both, a read-back half way must leave the trajectory
+    unchanged."""
     steps = 120
-    print("seam case 130x12x40 precision", precision, "staged" if staged else "", flush=True)
+    print("seam case 130x12x40 precision", precision, variant, flush=True)
     fields = sphere_fields(SEAM, seed=21, **SEAM_FIELDS)
     monkeypatch.setenv("MPLB_FUSED_KERNEL", "twopass")
     two, _ = pair(SEAM, fields, precision=precision, ht=False, fused=True)
     monkeypatch.delenv("MPLB_FUSED_KERNEL")
-    if staged:
-        monkeypatch.setenv("MPLB_MARCH_LDS", "1")
+    if variant:
+        monkeypatch.setenv(VARIANT_ENV[variant], "1")
     one, o = pair(SEAM, fields, precision=precision, ht=False, fused=True)
     assert one.get_info().fused_active == 1
     assert two.step(steps) == steps
-    if staged:
+    if variant:
         assert one.step(steps // 2) == steps // 2
         one.get("rho")
         assert one.step(steps - steps // 2) == steps - steps // 2
