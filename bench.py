@@ -936,9 +936,14 @@ def measure(args, thermal):
     # isothermal periodic D3Q15 (FP32 or FP16 storage) runs the single-pass kernel (psi stays
     # on chip): one timed kernel, where the two-pass kernels also report their first pass
     single = kname == "fused" and not thermal and "fused_aux" not in r["kern"]
+    # thermal periodic D3Q15 with FP32 storage runs the thermal single pass (one timed
+    # kernel: f and g pulled once, T and U of the previous step read once)
+    tsingle = kname == "fused" and thermal and "fused_aux" not in r["kern"]
     s_ref = 2 * 2 if args.precision == "fp16n" else 0             # FP16N: node reference in/out
     if kname == "collide":
         per_unit = 2 * 15 * s_mac + (1 + 3 + 3 + 1) * s_mac      # f in/out; rho, U, Fi, psi in
+    elif tsingle:
+        per_unit = 4 * 15 * s_pdf + 2 * (1 + 3) * s_mac + s_ref   # f, g in/out; T, U in and out
     elif thermal:
         per_unit = 4 * 15 * s_pdf + (1 + 1 + 1 + 3) * s_mac + s_ref   # f, g in/out; psi, T, S in; U out
     elif single:
@@ -955,7 +960,8 @@ def measure(args, thermal):
     march_name = "k_fused_march_lds" if staged else ("k_fused_march_pf" if ahead else "k_fused_march")
     variant = "_staged" if staged else ("_ahead" if ahead else "")
     key = "d3q15_%d_%s_%s" % (args.size, args.precision,
-                              "thermal" if thermal else ("iso" + variant if single else "iso_twopass"))
+                              ("thermal_single" if tsingle else "thermal") if thermal else
+                              ("iso" + variant if single else "iso_twopass"))
     tpath = os.path.join(ROOT, "profiles", "fused_traffic.json")
     if kname == "fused" and os.path.exists(tpath):
         rec = json.load(open(tpath)).get("kernels", {}).get(key)
@@ -985,8 +991,8 @@ def measure(args, thermal):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None,
-                     "traffic": traffic, "kernel": (march_name if single else ("k_fused_collide" if kname == "fused"
-                                                                                                else "k_collide_*")),
+                     "traffic": traffic, "kernel": (march_name if single else ("k_fused_tmarch" if tsingle else
+                                                                    ("k_fused_collide" if kname == "fused" else "k_collide_*"))),
                      "kernel_ms": round(k_ms, 5),
                      "bytes_per_unit": per_unit, "units_per_launch": N},
         "kernels_ms": {k: round(v[0], 5) for k, v in r["kern"].items()},

@@ -764,6 +764,48 @@ __device__ inline float march_psi(const FArgs& a, int x, int y, int z, float* v)
     return fpot(a, rho, T);
 }
 
+// Pulls split into issue and use, so that a lane's pulls for several sites are in flight
+// together: issue_pulls loads the stored values of node (x, y, z)'s pulled populations
+// (and, for f stored against per-node references, the references of their source nodes)
+// into registers without using them; stored_pull turns one into the population as
+// march_pull_from does.  FP16 values are held zero-extended in 32-bit registers: held as
+// halves, the compiler packs two per register right after the loads, which waits for
+// them.
+template <typename S>
+using raw_t = typename std::conditional<sizeof(S) == 2, unsigned, float>::type;
+
+__device__ inline float half_bits(unsigned u) { return (float)__builtin_bit_cast(_Float16, (unsigned short)u); }
+
+template <class L, typename S, bool REF>
+__device__ inline void issue_pulls(const FArgs& a, const void* src, int x, int y, int z, raw_t<S>* v, unsigned* r) {
+    const unsigned nx = (unsigned)a.nx, nxy = (unsigned)a.nxy;
+    const unsigned X[3] = {(unsigned)wrapd(x - 1, a.nx), (unsigned)x, (unsigned)wrapd(x + 1, a.nx)};
+    const unsigned Y[3] = {(unsigned)wrapd(y - 1, a.ny) * nx, (unsigned)y * nx, (unsigned)wrapd(y + 1, a.ny) * nx};
+    const unsigned Z[3] = {(unsigned)wrapd(z - 1, a.nz) * nxy, (unsigned)z * nxy, (unsigned)wrapd(z + 1, a.nz) * nxy};
+#pragma unroll
+    for (int q = 0; q < L::Q; q++) {
+        const unsigned up = X[1 - L::t.c[q][0]] + Y[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]];
+        if constexpr (sizeof(S) == 2) {
+            v[q] = ((const unsigned short*)src)[(size_t)q * a.P + up];
+        } else {
+            v[q] = ((const float*)src)[(size_t)q * a.P + up];
+        }
+        if constexpr (REF && kNodeRef<S>) r[q] = ((const unsigned short*)a.rin)[up];
+    }
+}
+
+template <class L, typename S, bool REF>
+__device__ inline float stored_pull(raw_t<S> v, const unsigned* r, int q, const float* shift) {
+    if constexpr (REF && kNodeRef<S>) {
+        const float wref = (float)L::t.omega[q] * half_bits(r[q]);
+        return fmaf(half_bits(v), wref, wref);
+    } else if constexpr (sizeof(S) == 2) {
+        return half_bits(v) + shift[q];
+    } else {
+        return v;
+    }
+}
+
 // The populations pulled for psi at plane z + 1 stay in registers (fn) until plane z + 1
 // collides; the register budget is MPLB_MARCH_WAVES waves per SIMD (three: about 150
 // VGPRs, no spills; four waves spill and run 13% slower on the 256^3 bubble).
@@ -878,7 +920,8 @@ template <class L, typename S, bool OUT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPLB_MARCH_PF_WAVES)))
 k_fused_march_pf(FArgs a) {
     static_assert(!kNodeRef<S>, "FP32 or FP16S storage");
-    using R = typename std::conditional<sizeof(S) == 2, _Float16, float>::type;
+    using R = raw_t<S>;
+    using M = typename std::conditional<sizeof(S) == 2, unsigned short, float>::type;   // stored element
     __shared__ float ring[4][MSLOT];
     const unsigned wi = (blockIdx.x & 7u) * (unsigned)a.mper + (blockIdx.x >> 3);
     if (wi >= (unsigned)a.mitems) return;
@@ -920,23 +963,20 @@ k_fused_march_pf(FArgs a) {
         if (edge) {
 #pragma unroll
             for (int q = 0; q < L::Q; q++)
-                pe[q] = ((const R*)a.fin)[(size_t)q * a.P +
+                pe[q] = ((const M*)a.fin)[(size_t)q * a.P +
                                           (EX[1 - L::t.c[q][0]] + EY[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]])];
             if (!a.Tuniform) te = a.Told[(int64_t)ex + (int64_t)ey * a.nx + (int64_t)zw * a.nxy];
         }
         if (ring_site) {
 #pragma unroll
             for (int q = 0; q < L::Q; q++)
-                pr[q] = ((const R*)a.fin)[(size_t)q * a.P +
+                pr[q] = ((const M*)a.fin)[(size_t)q * a.P +
                                           (RX[1 - L::t.c[q][0]] + RY[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]])];
             if (!a.Tuniform) tr = a.Told[(int64_t)rx + (int64_t)ry * a.nx + (int64_t)zw * a.nxy];
         }
         __builtin_amdgcn_s_setprio(0);
     };
-    auto stored = [&](R v, int q) -> float {
-        if constexpr (sizeof(S) == 2) return (float)v + a.fshift[q];
-        else return v;
-    };
+    auto stored = [&](R v, int q) -> float { return stored_pull<L, S, false>(v, nullptr, q, a.fshift); };
     // psi of raw plane zr from the landed pulls into its ring slot; column populations -> fo.
     // It ends with an explicit wait for every outstanding memory operation (the pulls, and
     // the previous plane's stores, issued before this plane's collision): the compiler
@@ -1100,53 +1140,105 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
     float f0[L::Q], f1[L::Q], f2[L::Q], g0[L::Q], g1[L::Q], tmp[L::Q];
     float s0 = 0.f, T0 = 0.f, s1 = 0.f, T1 = 0.f;
 
-    // rho of the pulled f, previous T and U of site (sx, sy) of the two-site plane zr
-    auto load_site = [&](int sx, int sy, int zr, float* v) {
-        const int gx = wrapd(x0 - 2 + sx, a.nx), gy = wrapd(y0 - 2 + sy, a.ny), gz = wrapd(zr, a.nz);
-        const float rho = march_pull_from<L, S, kNodeRef<S>>(a, a.fin, a.fshift, gx, gy, gz, v);
-        const int64_t id = (int64_t)gx + (int64_t)gy * a.nx + (int64_t)gz * a.nxy;
-        const int sl = tslot(zr), i = sy * TRX + sx;
-        s_rho[sl][i] = rho;
-        s_T[sl][i] = a.Told[id];
-        s_U[sl][0][i] = a.Uold[id];
-        s_U[sl][1][i] = a.Uold[N + id];
-        s_U[sl][2][i] = a.Uold[2 * N + id];
-    };
-    // step 1 for plane zr; the own site's populations go to f2
+    // step 1 for plane zr: rho of the pulled f and the previous T and U of the column's
+    // sites and of the two-site ring into the LDS planes; the own site's populations go to
+    // f2.  Every pull of the lane (own site and ring site) and its T and U loads are issued
+    // before any of them is used, so the plane costs one memory round trip.
+    constexpr int QR = kNodeRef<S> ? L::Q : 1;
     auto load_plane = [&](int zr) {
+        const int gz = wrapd(zr, a.nz), sl = tslot(zr);
+        raw_t<S> po[L::Q], pr[L::Q];
+        unsigned ro[QR], rr[QR];
+        float To = 0.f, Uo0 = 0.f, Uo1 = 0.f, Uo2 = 0.f, Tr = 0.f, Ur0 = 0.f, Ur1 = 0.f, Ur2 = 0.f;
+        const int ogx = wrapd(x0 + tx, a.nx), ogy = wrapd(y0 + ty, a.ny);
+        const int rgx = wrapd(x0 - 2 + r2x, a.nx), rgy = wrapd(y0 - 2 + r2y, a.ny);
+        const int64_t ido = (int64_t)ogx + (int64_t)ogy * a.nx + (int64_t)gz * a.nxy;
+        const int64_t idr = (int64_t)rgx + (int64_t)rgy * a.nx + (int64_t)gz * a.nxy;
         __builtin_amdgcn_s_setprio(MPLB_PULL_PRIO);
-        if (own2) load_site(tx + 2, ty + 2, zr, f2);
-        if (r2v) load_site(r2x, r2y, zr, tmp);
+        if (own2) {
+            issue_pulls<L, S, true>(a, a.fin, ogx, ogy, gz, po, ro);
+            To = a.Told[ido];
+            Uo0 = a.Uold[ido];
+            Uo1 = a.Uold[N + ido];
+            Uo2 = a.Uold[2 * N + ido];
+        }
+        if (r2v) {
+            issue_pulls<L, S, true>(a, a.fin, rgx, rgy, gz, pr, rr);
+            Tr = a.Told[idr];
+            Ur0 = a.Uold[idr];
+            Ur1 = a.Uold[N + idr];
+            Ur2 = a.Uold[2 * N + idr];
+        }
         __builtin_amdgcn_s_setprio(0);
+        if (own2) {
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) f2[q] = stored_pull<L, S, true>(po[q], ro, q, a.fshift);
+            const int i = (ty + 2) * TRX + tx + 2;
+            s_rho[sl][i] = rho_sym<L, float>(f2);
+            s_T[sl][i] = To;
+            s_U[sl][0][i] = Uo0;
+            s_U[sl][1][i] = Uo1;
+            s_U[sl][2][i] = Uo2;
+        }
+        if (r2v) {
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) tmp[q] = stored_pull<L, S, true>(pr[q], rr, q, a.fshift);
+            const int i = r2y * TRX + r2x;
+            s_rho[sl][i] = rho_sym<L, float>(tmp);
+            s_T[sl][i] = Tr;
+            s_U[sl][0][i] = Ur0;
+            s_U[sl][1][i] = Ur1;
+            s_U[sl][2][i] = Ur2;
+        }
+        // nothing of this phase is in flight any more; saying so keeps the compiler (which
+        // cannot pair a lane's skipped loads with its skipped uses) from waiting on the
+        // collision's stores when it next reuses these registers
+        __builtin_amdgcn_s_waitcnt(kWaitVm);
     };
-    // heat source, new temperature and psi of site (px, py) of the one-site plane zr; the
-    // pulled g populations are left in v
-    auto psi_site = [&](int px, int py, int zr, float* v, float& sv, float& Tv) {
+    // heat source of site (px, py) of the one-site plane zr from the rho, T and U stencils
+    // in LDS
+    auto heat_site = [&](int px, int py, int zr, float& rho) {
         const int sl = tslot(zr), sm = tslot(zr - 1), sp = tslot(zr + 1);
         const int i = (py + 1) * TRX + px + 1;
-        const float rho = s_rho[sl][i], Told = s_T[sl][i];
+        rho = s_rho[sl][i];
+        const float Told = s_T[sl][i];
         float dot = 0.f, divU = 0.f;
         heat_axis(s_T[sl][i + 1], s_T[sl][i - 1], s_rho[sl][i + 1], s_rho[sl][i - 1], s_U[sl][0][i + 1],
                   s_U[sl][0][i - 1], dot, divU);
         heat_axis(s_T[sl][i + TRX], s_T[sl][i - TRX], s_rho[sl][i + TRX], s_rho[sl][i - TRX], s_U[sl][1][i + TRX],
                   s_U[sl][1][i - TRX], dot, divU);
         heat_axis(s_T[sp][i], s_T[sm][i], s_rho[sp][i], s_rho[sm][i], s_U[sp][2][i], s_U[sm][2][i], dot, divU);
-        const float s = heat_tail<L>(a, rho, Told, dot, divU);
-        const int gx = wrapd(x0 - 1 + px, a.nx), gy = wrapd(y0 - 1 + py, a.ny), gz = wrapd(zr, a.nz);
-        const float sg = march_pull_from<L, S, false>(a, a.gin, a.gshift, gx, gy, gz, v);
-        const float T = sg + 0.5f * s;
-        s_psi[sl][py * TPX + px] = fpot(a, rho, T);
-        sv = s;
-        Tv = T;
+        return heat_tail<L>(a, rho, Told, dot, divU);
     };
-    // step 2 for plane zr; the own site's g populations, source and temperature go to g1,
-    // s1, T1
+    // step 2 for plane zr: heat source, new temperature and psi of the column's sites and
+    // of the one-site ring; the own site's g populations, source and temperature go to g1,
+    // s1, T1.  The lane's g pulls (own and ring site) are issued before the heat sources
+    // are formed, so their latency runs under that arithmetic.
     auto psi_plane = [&](int zr) {
-        float sd, Td;
+        const int gz = wrapd(zr, a.nz), sl = tslot(zr);
+        raw_t<S> go[L::Q], gr[L::Q];
         __builtin_amdgcn_s_setprio(MPLB_PULL_PRIO);
-        if (own1) psi_site(tx + 1, ty + 1, zr, g1, s1, T1);
-        if (r1v) psi_site(r1x, r1y, zr, tmp, sd, Td);
+        if (own1) issue_pulls<L, S, false>(a, a.gin, wrapd(x0 + tx, a.nx), wrapd(y0 + ty, a.ny), gz, go, nullptr);
+        if (r1v) issue_pulls<L, S, false>(a, a.gin, wrapd(x0 - 1 + r1x, a.nx), wrapd(y0 - 1 + r1y, a.ny), gz, gr, nullptr);
         __builtin_amdgcn_s_setprio(0);
+        float rho_o = 0.f, rho_r = 0.f, so = 0.f, sr = 0.f;
+        if (own1) so = heat_site(tx + 1, ty + 1, zr, rho_o);
+        if (r1v) sr = heat_site(r1x, r1y, zr, rho_r);
+        if (own1) {
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) g1[q] = stored_pull<L, S, false>(go[q], nullptr, q, a.gshift);
+            const float T = rho_sym<L, float>(g1) + 0.5f * so;
+            s_psi[sl][(ty + 1) * TPX + tx + 1] = fpot(a, rho_o, T);
+            s1 = so;
+            T1 = T;
+        }
+        if (r1v) {
+#pragma unroll
+            for (int q = 0; q < L::Q; q++) tmp[q] = stored_pull<L, S, false>(gr[q], nullptr, q, a.gshift);
+            const float T = rho_sym<L, float>(tmp) + 0.5f * sr;
+            s_psi[sl][r1y * TPX + r1x] = fpot(a, rho_r, T);
+        }
+        __builtin_amdgcn_s_waitcnt(kWaitVm);
     };
 
     load_plane(z0 - 2);
@@ -1986,11 +2078,14 @@ int fused_create(mplb_case* c) {
         const bool periodic3 = !twopass && m.lbmodel == 1 && z->face_table == 0 && !any_bnd && m.nx >= 3 && m.ny >= 3 &&
                                m.nz >= 3;
         z->march = periodic3 && !z->thermal;
-        // thermal single pass: xuMRT f with myMRT g (the D3Q15 thermal pair)
-        // (opt-in, MPLB_THERMAL_MARCH=1: it reads f and g once per step but runs slower than
-        // the three passes, 1.48 against 1.38 ms per step on the 256^3 thermal bubble)
+        // thermal single pass: xuMRT f with myMRT g (the D3Q15 thermal pair).  The default
+        // with FP32 storage (1.285 against 1.383 ms per step for the three passes on the
+        // 256^3 thermal bubble); with FP16 storage the three passes are faster (0.985
+        // against 1.29 ms).  MPLB_THERMAL_MARCH=1 / =0 forces the single pass / the three
+        // passes.
         const char* tenv = std::getenv("MPLB_THERMAL_MARCH");
-        z->tmarch = periodic3 && z->thermal && c->fc[1].mp.model == M_MYMRT && tenv && std::strcmp(tenv, "1") == 0;
+        const bool tpick = tenv ? std::strcmp(tenv, "1") == 0 : !z->half;
+        z->tmarch = periodic3 && z->thermal && c->fc[1].mp.model == M_MYMRT && tpick;
         // LDS-staged pulls for the isothermal single pass, FP32 storage (opt-in, MPLB_MARCH_LDS=1)
         const char* lenv = std::getenv("MPLB_MARCH_LDS");
         z->march_lds = z->march && !z->half && lenv && std::strcmp(lenv, "1") == 0;

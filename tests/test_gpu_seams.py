@@ -94,6 +94,7 @@ def test_thermal_march_tracks_three_passes(precision, monkeypatch):
     steps = 80
     print("thermal seam case 130x10x24 precision", precision, flush=True)
     fields = sphere_fields(TSEAM, seed=23, **TSEAM_FIELDS)
+    monkeypatch.setenv("MPLB_THERMAL_MARCH", "0")
     three, _ = pair(TSEAM, fields, precision=precision, ht=True, fused=True)
     monkeypatch.setenv("MPLB_THERMAL_MARCH", "1")
     one, o = pair(TSEAM, fields, precision=precision, ht=True, fused=True)
