@@ -857,13 +857,44 @@ __global__ void __launch_bounds__(256)
     // A wave pulling a plane runs at raised priority (s_setprio), so the SIMD issues its
     // loads ahead of other waves' collision arithmetic and more loads are in flight
     // (8% faster on the 256^3 bubble).
+    // FP16S issues the lane's column and ring pulls of a plane together (0.330 against
+    // 0.340 ms per step on the 256^3 bubble); FP32 and FP16N pull and use the column site
+    // first (together: 0.468 against 0.446, and 0.467 against 0.434)
     auto plane = [&](int zr) {
-        const int zw = wrapd(zr, a.nz);
-        float* sl = ring[(zr - z0 + 1) & 3];
-        __builtin_amdgcn_s_setprio(MPLB_PULL_PRIO);
-        if (edge) sl[(ty + 1) * MHX + tx + 1] = march_psi<L, S>(a, ex, ey, zw, fn);
-        if (ring_site) sl[rhy * MHX + rhx] = march_psi<L, S>(a, rx, ry, zw, tmp);
-        __builtin_amdgcn_s_setprio(0);
+        if constexpr (std::is_same<S, _Float16>::value) {
+            const int zw = wrapd(zr, a.nz);
+            float* sl = ring[(zr - z0 + 1) & 3];
+            raw_t<S> pe[L::Q], pr[L::Q];
+            float te = a.Tconst, tr = a.Tconst;
+            __builtin_amdgcn_s_setprio(MPLB_PULL_PRIO);
+            if (edge) {
+                issue_pulls<L, S, false>(a, a.fin, ex, ey, zw, pe, nullptr);
+                if (!a.Tuniform) te = a.Told[(int64_t)ex + (int64_t)ey * a.nx + (int64_t)zw * a.nxy];
+            }
+            if (ring_site) {
+                issue_pulls<L, S, false>(a, a.fin, rx, ry, zw, pr, nullptr);
+                if (!a.Tuniform) tr = a.Told[(int64_t)rx + (int64_t)ry * a.nx + (int64_t)zw * a.nxy];
+            }
+            __builtin_amdgcn_s_setprio(0);
+            if (edge) {
+#pragma unroll
+                for (int q = 0; q < L::Q; q++) fn[q] = stored_pull<L, S, false>(pe[q], nullptr, q, a.fshift);
+                sl[(ty + 1) * MHX + tx + 1] = fpot(a, rho_sym<L, float>(fn), te);
+            }
+            if (ring_site) {
+#pragma unroll
+                for (int q = 0; q < L::Q; q++) tmp[q] = stored_pull<L, S, false>(pr[q], nullptr, q, a.fshift);
+                sl[rhy * MHX + rhx] = fpot(a, rho_sym<L, float>(tmp), tr);
+            }
+            __builtin_amdgcn_s_waitcnt(kWaitVm);
+        } else {
+            const int zw = wrapd(zr, a.nz);
+            float* sl = ring[(zr - z0 + 1) & 3];
+            __builtin_amdgcn_s_setprio(MPLB_PULL_PRIO);
+            if (edge) sl[(ty + 1) * MHX + tx + 1] = march_psi<L, S>(a, ex, ey, zw, fn);
+            if (ring_site) sl[rhy * MHX + rhx] = march_psi<L, S>(a, rx, ry, zw, tmp);
+            __builtin_amdgcn_s_setprio(0);
+        }
     };
     plane(z0 - 1);
     plane(z0);
@@ -1137,21 +1168,25 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
     const bool own2 = x <= a.nx + 1 && y <= a.ny + 1;
     const bool own1 = x <= a.nx && y <= a.ny;
 
-    float f0[L::Q], f1[L::Q], f2[L::Q], g0[L::Q], g1[L::Q], tmp[L::Q];
+    // the own site's f of the planes waiting for their collision (each thread reads only
+    // the slots it wrote)
+    __shared__ float s_f[2][L::Q][TNT];
+    float f0[L::Q], g0[L::Q], g1[L::Q], tmp[L::Q];
     float s0 = 0.f, T0 = 0.f, s1 = 0.f, T1 = 0.f;
 
-    // step 1 for plane zr: rho of the pulled f and the previous T and U of the column's
-    // sites and of the two-site ring into the LDS planes; the own site's populations go to
-    // f2.  Every pull of the lane (own site and ring site) and its T and U loads are issued
-    // before any of them is used, so the plane costs one memory round trip.
+    // step 1 for plane zr, in two halves: issue_plane loads the pulled f of the column's
+    // sites and of the two-site ring with their previous T and U (every load of the lane
+    // issued before any is used); finish_plane turns them into rho, T and U in the LDS
+    // planes and the own site's populations into its s_f slot.  In the march the loads of
+    // plane z + 3 are issued before plane z collides, so they land under the collision.
     constexpr int QR = kNodeRef<S> ? L::Q : 1;
-    auto load_plane = [&](int zr) {
-        const int gz = wrapd(zr, a.nz), sl = tslot(zr);
-        raw_t<S> po[L::Q], pr[L::Q];
-        unsigned ro[QR], rr[QR];
-        float To = 0.f, Uo0 = 0.f, Uo1 = 0.f, Uo2 = 0.f, Tr = 0.f, Ur0 = 0.f, Ur1 = 0.f, Ur2 = 0.f;
-        const int ogx = wrapd(x0 + tx, a.nx), ogy = wrapd(y0 + ty, a.ny);
-        const int rgx = wrapd(x0 - 2 + r2x, a.nx), rgy = wrapd(y0 - 2 + r2y, a.ny);
+    raw_t<S> po[L::Q], pr[L::Q];
+    unsigned ro[QR], rr[QR];
+    float To = 0.f, Uo0 = 0.f, Uo1 = 0.f, Uo2 = 0.f, Tr = 0.f, Ur0 = 0.f, Ur1 = 0.f, Ur2 = 0.f;
+    const int ogx = wrapd(x0 + tx, a.nx), ogy = wrapd(y0 + ty, a.ny);
+    const int rgx = wrapd(x0 - 2 + r2x, a.nx), rgy = wrapd(y0 - 2 + r2y, a.ny);
+    auto issue_plane = [&](int zr) __attribute__((always_inline)) {
+        const int gz = wrapd(zr, a.nz);
         const int64_t ido = (int64_t)ogx + (int64_t)ogy * a.nx + (int64_t)gz * a.nxy;
         const int64_t idr = (int64_t)rgx + (int64_t)rgy * a.nx + (int64_t)gz * a.nxy;
         __builtin_amdgcn_s_setprio(MPLB_PULL_PRIO);
@@ -1170,9 +1205,16 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
             Ur2 = a.Uold[2 * N + idr];
         }
         __builtin_amdgcn_s_setprio(0);
+    };
+    auto finish_plane = [&](int zr) __attribute__((always_inline)) {
+        const int sl = tslot(zr);
         if (own2) {
+            float f2[L::Q];
 #pragma unroll
-            for (int q = 0; q < L::Q; q++) f2[q] = stored_pull<L, S, true>(po[q], ro, q, a.fshift);
+            for (int q = 0; q < L::Q; q++) {
+                f2[q] = stored_pull<L, S, true>(po[q], ro, q, a.fshift);
+                s_f[zr & 1][q][t] = f2[q];
+            }
             const int i = (ty + 2) * TRX + tx + 2;
             s_rho[sl][i] = rho_sym<L, float>(f2);
             s_T[sl][i] = To;
@@ -1195,9 +1237,13 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
         // collision's stores when it next reuses these registers
         __builtin_amdgcn_s_waitcnt(kWaitVm);
     };
+    auto load_plane = [&](int zr) __attribute__((always_inline)) {
+        issue_plane(zr);
+        finish_plane(zr);
+    };
     // heat source of site (px, py) of the one-site plane zr from the rho, T and U stencils
     // in LDS
-    auto heat_site = [&](int px, int py, int zr, float& rho) {
+    auto heat_site = [&](int px, int py, int zr, float& rho) __attribute__((always_inline)) {
         const int sl = tslot(zr), sm = tslot(zr - 1), sp = tslot(zr + 1);
         const int i = (py + 1) * TRX + px + 1;
         rho = s_rho[sl][i];
@@ -1214,7 +1260,7 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
     // of the one-site ring; the own site's g populations, source and temperature go to g1,
     // s1, T1.  The lane's g pulls (own and ring site) are issued before the heat sources
     // are formed, so their latency runs under that arithmetic.
-    auto psi_plane = [&](int zr) {
+    auto psi_plane = [&](int zr) __attribute__((always_inline)) {
         const int gz = wrapd(zr, a.nz), sl = tslot(zr);
         raw_t<S> go[L::Q], gr[L::Q];
         __builtin_amdgcn_s_setprio(MPLB_PULL_PRIO);
@@ -1244,14 +1290,10 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
     load_plane(z0 - 2);
     load_plane(z0 - 1);
     load_plane(z0);
-#pragma unroll
-    for (int q = 0; q < L::Q; q++) f0[q] = f2[q];
     __syncthreads();
     psi_plane(z0 - 1);
     __syncthreads();
     load_plane(z0 + 1);
-#pragma unroll
-    for (int q = 0; q < L::Q; q++) f1[q] = f2[q];
     __syncthreads();
     psi_plane(z0);
 #pragma unroll
@@ -1259,11 +1301,16 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
     s0 = s1;
     T0 = T1;
     __syncthreads();
+    issue_plane(z0 + 2);
     for (int z = z0; z < z1; z++) {
-        load_plane(z + 2);
+        // plane z's populations leave their slot before plane z + 2's take it
+#pragma unroll
+        for (int q = 0; q < L::Q; q++) f0[q] = s_f[z & 1][q][t];
+        finish_plane(z + 2);
         __syncthreads();
         psi_plane(z + 1);
         __syncthreads();
+        if (z + 3 <= z1 + 1) issue_plane(z + 3);
         if (owner) {
             const float rho = rho_sym<L, float>(f0);
             float psin[L::Q];
@@ -1301,11 +1348,7 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
             }
         }
 #pragma unroll
-        for (int q = 0; q < L::Q; q++) {
-            f0[q] = f1[q];
-            f1[q] = f2[q];
-            g0[q] = g1[q];
-        }
+        for (int q = 0; q < L::Q; q++) g0[q] = g1[q];
         s0 = s1;
         T0 = T1;
     }
