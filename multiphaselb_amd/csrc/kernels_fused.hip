@@ -196,10 +196,17 @@ struct NodePos {
 // psi = potential(rho, T) (potential.c:7-13) in FP32 with the loop-invariant factors taken
 // from the host for the van der Waals EOS (one division instead of three); other EOS
 // types use the shared form.
+// The fused FP32 kernels take quotients and square roots from the hardware reciprocal
+// and square root (about 1 ulp) instead of the correctly rounded instruction sequences
+// (about ten instructions each): the single passes spend half their time issuing VALU
+// instructions, and their results carry FP32 rounding anyway.
+__device__ inline float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 __device__ inline float fpot(const FArgs& a, float rho, float T) {
     if (a.vdw) {
-        const float p = rho * a.vR * T / (a.vM - rho * a.vb) - a.va_m2 * rho * rho;
-        return sqrtf(2.f * (p - rho * float(kCs2)) * a.vinv_g);
+        const float p = rho * a.vR * T * rcp_fast(a.vM - rho * a.vb) - a.va_m2 * rho * rho;
+        return sqrt_fast(2.f * (p - rho * float(kCs2)) * a.vinv_g);
     }
     return potential<float>(a.eos, rho, T);
 }
@@ -369,21 +376,28 @@ __device__ inline float heat_tail(const FArgs& a, float rho, float Told, float d
     const float Cv = (float)a.eos.Cv;
     float lambda;
     if (a.gmodel == 1) {
-        lambda = (L::D == 2) ? rho * Cv * (1.0f / a.gl[3] - 0.5f) * (4.0f + 3.0f * a.a1 + 2.0f * a.a2) / 6.0f
-                             : rho * Cv * (1.0f / a.gl[3] - 0.5f) * (6.0f + 11.0f * a.a1 + a.a2) / 9.0f;
+        lambda = (L::D == 2) ? rho * Cv * (rcp_fast(a.gl[3]) - 0.5f) * (4.0f + 3.0f * a.a1 + 2.0f * a.a2) / 6.0f
+                             : rho * Cv * (rcp_fast(a.gl[3]) - 0.5f) * (6.0f + 11.0f * a.a1 + a.a2) / 9.0f;
     } else {
-        lambda = rho * Cv * (1.0f / a.gl[3] - 0.5f);
+        lambda = rho * Cv * (rcp_fast(a.gl[3]) - 0.5f);
     }
-    float dpdT = p_eos<float>(a.eos, rho, 1.1f * Told) - p_eos<float>(a.eos, rho, 0.9f * Told);
-    dpdT = dpdT / (0.2f * Told);
-    return -lambda * dot / Cv + divU * Told * (1.0f - dpdT / (rho * Cv));
+    // dp/dT by the central difference of heatSource.c; the van der Waals pressure is linear
+    // in T, so there the difference quotient is rho R / (M - rho b) exactly
+    float dpdT;
+    if (a.vdw) {
+        dpdT = rho * a.vR * rcp_fast(a.vM - rho * a.vb);
+    } else {
+        dpdT = p_eos<float>(a.eos, rho, 1.1f * Told) - p_eos<float>(a.eos, rho, 0.9f * Told);
+        dpdT = dpdT * rcp_fast(0.2f * Told);
+    }
+    return -lambda * dot * rcp_fast(Cv) + divU * Told * (1.0f - dpdT * rcp_fast(rho * Cv));
 }
 
 // central-difference terms of one axis (heatSource.c:60-80 with both neighbours present)
 __device__ inline void heat_axis(float Tp, float Tm, float rp, float rm, float up, float um, float& dot, float& divU) {
 #pragma clang fp contract(on)
     const float gT = 0.5f * (Tp - Tm);
-    const float gR = 0.5f * (1.0f / rp - 1.0f / rm);
+    const float gR = 0.5f * (rcp_fast(rp) - rcp_fast(rm));
     const float dU = 0.5f * (up - um);
     dot += gT * gR;
     divU += dU;
@@ -478,7 +492,7 @@ __device__ inline void collide_node(const FArgs& a, float* v, float rho, const f
         Ft[d] = Fi[d] + grav + (interior ? (float)a.eos.fext[d] : 0.f);
         Fu[d] = a.fmodel == 0 ? Ft[d] : Fi[d] + grav;
     }
-    const float irho = 1.f / rho;
+    const float irho = rcp_fast(rho);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         float j = 0.f;
@@ -508,7 +522,7 @@ __device__ inline void collide_node(const FArgs& a, float* v, float rho, const f
         const float uF = U[0] * Fu[0] + U[1] * Fu[1];
         const float fi2 = Fi[0] * Fi[0] + Fi[1] * Fi[1];
         Sf[0] = 0.f;
-        const float fp = fi2 / psi2;
+        const float fp = fi2 * rcp_fast(psi2);
         Sf[1] = 6.f * uF + a.sf1 * fp;
         Sf[2] = -6.f * uF - a.sf2 * fp;
         Sf[3] = Fu[0];
@@ -536,7 +550,7 @@ __device__ inline void collide_node(const FArgs& a, float* v, float rho, const f
         const float uF = U[0] * Fu[0] + U[1] * Fu[1] + U[2] * Fu[2];
         const float F2 = Fu[0] * Fu[0] + Fu[1] * Fu[1] + Fu[2] * Fu[2];
         Sf[0] = 0.f;
-        Sf[1] = 2.f * uF + a.sf1 * F2 / psi2;
+        Sf[1] = 2.f * uF + a.sf1 * F2 * rcp_fast(psi2);
         Sf[2] = -10.f * uF;
         Sf[3] = Fu[0];
         Sf[4] = (-7.f / 3.f) * Fu[0];
