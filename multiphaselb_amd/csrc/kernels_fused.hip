@@ -744,6 +744,12 @@ constexpr int kWaitAll = 0;
 constexpr int kWaitLds = 0xF | 0x7 << 4 | 3 << 14;
 constexpr int kWaitVm = 0x7 << 4 | 0xF << 8;
 
+// Element index of population q of node i in 32 bits, for the thermal single pass: with
+// 64-bit plane offsets the compiler keeps thirty 64-bit plane addresses (f and g) in
+// scalar registers, which spill and come back through v_readlane before every pull.  The
+// thermal single pass runs only on boxes whose Q x pitch is below 2^32.
+__device__ inline unsigned pidx(const FArgs& a, int q, unsigned i) { return (unsigned)q * (unsigned)a.P + i; }
+
 // populations of node (x, y, z) pulled from x - c_q (duplicate-node periodic wrap) and
 // their sum (REF: f stored against per-node references, FP16N)
 template <class L, typename S, bool REF>
@@ -790,7 +796,7 @@ using raw_t = typename std::conditional<sizeof(S) == 2, unsigned, float>::type;
 
 __device__ inline float half_bits(unsigned u) { return (float)__builtin_bit_cast(_Float16, (unsigned short)u); }
 
-template <class L, typename S, bool REF>
+template <class L, typename S, bool REF, bool I32 = false>
 __device__ inline void issue_pulls(const FArgs& a, const void* src, int x, int y, int z, raw_t<S>* v, unsigned* r) {
     const unsigned nx = (unsigned)a.nx, nxy = (unsigned)a.nxy;
     const unsigned X[3] = {(unsigned)wrapd(x - 1, a.nx), (unsigned)x, (unsigned)wrapd(x + 1, a.nx)};
@@ -799,10 +805,12 @@ __device__ inline void issue_pulls(const FArgs& a, const void* src, int x, int y
 #pragma unroll
     for (int q = 0; q < L::Q; q++) {
         const unsigned up = X[1 - L::t.c[q][0]] + Y[1 - L::t.c[q][1]] + Z[1 - L::t.c[q][2]];
+        // I32: 32-bit element index (pidx), else a 64-bit plane offset plus the node
+        const int64_t e = I32 ? (int64_t)pidx(a, q, up) : (int64_t)q * a.P + up;
         if constexpr (sizeof(S) == 2) {
-            v[q] = ((const unsigned short*)src)[(size_t)q * a.P + up];
+            v[q] = ((const unsigned short*)src)[e];
         } else {
-            v[q] = ((const float*)src)[(size_t)q * a.P + up];
+            v[q] = ((const float*)src)[e];
         }
         if constexpr (REF && kNodeRef<S>) r[q] = ((const unsigned short*)a.rin)[up];
     }
@@ -1205,14 +1213,14 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
         const int64_t idr = (int64_t)rgx + (int64_t)rgy * a.nx + (int64_t)gz * a.nxy;
         __builtin_amdgcn_s_setprio(MPLB_PULL_PRIO);
         if (own2) {
-            issue_pulls<L, S, true>(a, a.fin, ogx, ogy, gz, po, ro);
+            issue_pulls<L, S, true, true>(a, a.fin, ogx, ogy, gz, po, ro);
             To = a.Told[ido];
             Uo0 = a.Uold[ido];
             Uo1 = a.Uold[N + ido];
             Uo2 = a.Uold[2 * N + ido];
         }
         if (r2v) {
-            issue_pulls<L, S, true>(a, a.fin, rgx, rgy, gz, pr, rr);
+            issue_pulls<L, S, true, true>(a, a.fin, rgx, rgy, gz, pr, rr);
             Tr = a.Told[idr];
             Ur0 = a.Uold[idr];
             Ur1 = a.Uold[N + idr];
@@ -1278,8 +1286,8 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
         const int gz = wrapd(zr, a.nz), sl = tslot(zr);
         raw_t<S> go[L::Q], gr[L::Q];
         __builtin_amdgcn_s_setprio(MPLB_PULL_PRIO);
-        if (own1) issue_pulls<L, S, false>(a, a.gin, wrapd(x0 + tx, a.nx), wrapd(y0 + ty, a.ny), gz, go, nullptr);
-        if (r1v) issue_pulls<L, S, false>(a, a.gin, wrapd(x0 - 1 + r1x, a.nx), wrapd(y0 - 1 + r1y, a.ny), gz, gr, nullptr);
+        if (own1) issue_pulls<L, S, false, true>(a, a.gin, wrapd(x0 + tx, a.nx), wrapd(y0 + ty, a.ny), gz, go, nullptr);
+        if (r1v) issue_pulls<L, S, false, true>(a, a.gin, wrapd(x0 - 1 + r1x, a.nx), wrapd(y0 - 1 + r1y, a.ny), gz, gr, nullptr);
         __builtin_amdgcn_s_setprio(0);
         float rho_o = 0.f, rho_r = 0.f, so = 0.f, sr = 0.f;
         if (own1) so = heat_site(tx + 1, ty + 1, zr, rho_o);
@@ -1338,14 +1346,14 @@ __global__ void __launch_bounds__(TNT) __attribute__((amdgpu_waves_per_eu(MPLB_T
                 float rinv;
                 const float R = put_ref<S>(a, id, rho, rinv);
 #pragma unroll
-                for (int q = 0; q < L::Q; q++) stf_ref<L>(a.fout, q * a.P + id, q, f0[q], R, rinv);
+                for (int q = 0; q < L::Q; q++) stf_ref<L>(a.fout, pidx(a, q, (unsigned)id), q, f0[q], R, rinv);
             } else {
 #pragma unroll
-                for (int q = 0; q < L::Q; q++) st<S>(a.fout, q * a.P + id, f0[q], a.fshift[q]);
+                for (int q = 0; q < L::Q; q++) st<S>(a.fout, pidx(a, q, (unsigned)id), f0[q], a.fshift[q]);
             }
             collide_g<L>(a, g0, T0, s0, U);
 #pragma unroll
-            for (int q = 0; q < L::Q; q++) st<S>(a.gout, q * a.P + id, g0[q], a.gshift[q]);
+            for (int q = 0; q < L::Q; q++) st<S>(a.gout, pidx(a, q, (unsigned)id), g0[q], a.gshift[q]);
             a.Tnew[id] = T0;
             a.Unew[id] = U[0];
             a.Unew[N + id] = U[1];
@@ -2142,7 +2150,9 @@ int fused_create(mplb_case* c) {
         // passes.
         const char* tenv = std::getenv("MPLB_THERMAL_MARCH");
         const bool tpick = tenv ? std::strcmp(tenv, "1") == 0 : !z->half;
-        z->tmarch = periodic3 && z->thermal && c->fc[1].mp.model == M_MYMRT && tpick;
+        // (it indexes populations in 32 bits: Q x pitch below 2^32)
+        z->tmarch = periodic3 && z->thermal && c->fc[1].mp.model == M_MYMRT && tpick &&
+                    15 * pitch_for(N) <= (int64_t)0xFFFFFFFF;
         // LDS-staged pulls for the isothermal single pass, FP32 storage (opt-in, MPLB_MARCH_LDS=1)
         const char* lenv = std::getenv("MPLB_MARCH_LDS");
         z->march_lds = z->march && !z->half && lenv && std::strcmp(lenv, "1") == 0;
