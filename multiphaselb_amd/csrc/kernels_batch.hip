@@ -196,6 +196,28 @@ __device__ inline void pull(const BArgs& a, const BCase& bc, const Pos& p, const
 
 // density of pulled populations: the reference's index order in FP64 (parity), the
 // inversion-symmetric pair form in FP32 (no drift from biased rounding)
+// FP32 quotients and the pseudopotential from the hardware reciprocal and square root
+// (about 1 ulp) instead of the correctly rounded instruction sequences; FP64 (the
+// parity mode) keeps IEEE division and sqrt
+template <typename T>
+__device__ inline T bdiv(T x, T y) {
+    if constexpr (sizeof(T) == 4) return x * __builtin_amdgcn_rcpf(y);
+    else return x / y;
+}
+
+template <typename T>
+__device__ inline T bpotential(const EosParams& e, T rho, T temp) {
+    if constexpr (sizeof(T) == 4) {
+        if (e.type == EOS_VDW) {
+            const float M = (float)e.M;
+            const float p = rho * (float)e.R * temp * __builtin_amdgcn_rcpf(M - rho * (float)e.b) -
+                            (float)e.a * __builtin_amdgcn_rcpf(M * M) * rho * rho;
+            return __builtin_amdgcn_sqrtf(2.f * (p - rho * (float)kCs2) * __builtin_amdgcn_rcpf((float)e.G));
+        }
+    }
+    return potential<T>(e, rho, temp);
+}
+
 template <typename T>
 __device__ inline T density(const T* v) {
     if constexpr (sizeof(T) == 8) {
@@ -234,7 +256,7 @@ __global__ void k_batch_psi(BArgs a) {
     if (!p.valid) return;
     T v[9];
     const T rho = load_state<S, T, MODE>(a, bc, p, v);
-    ((T*)a.psi)[p.gid] = potential<T>(a.eos, rho, T(bc.T));
+    ((T*)a.psi)[p.gid] = bpotential<T>(a.eos, rho, T(bc.T));
 }
 
 // Interaction force (interForce.c:16-39, interactionForce.c:47-79; every node of a fully
@@ -297,7 +319,7 @@ __device__ inline void collide_site(const BArgs& a, const BCase& bc, T* v, const
                     else if (c == -1) lv -= v[PR.q[k]] - v[PR.r[k]];
                 }
             }
-            U[d] = (lv + Ft[d] * T(0.5)) / rho;
+            U[d] = bdiv<T>(lv + Ft[d] * T(0.5), rho);
         }
     }
     T m[9], meq[9], Sf[9];
@@ -318,8 +340,8 @@ __device__ inline void collide_site(const BArgs& a, const BCase& bc, T* v, const
     const T uF = U[0] * Ft[0] + U[1] * Ft[1];
     const T fi2 = Fi[0] * Fi[0] + Fi[1] * Fi[1];
     Sf[0] = 0;
-    Sf[1] = T(6) * uF + T(12) * sigma * fi2 / (psic * psic * ((T(1) / L1) - T(0.5)));
-    Sf[2] = T(-6) * uF - T(12) * sigma * fi2 / (psic * psic * ((T(1) / L2) - T(0.5)));
+    Sf[1] = T(6) * uF + bdiv<T>(T(12) * sigma * fi2, psic * psic * (bdiv<T>(T(1), L1) - T(0.5)));
+    Sf[2] = T(-6) * uF - bdiv<T>(T(12) * sigma * fi2, psic * psic * (bdiv<T>(T(1), L2) - T(0.5)));
     Sf[3] = Ft[0];
     Sf[4] = -Ft[0];
     Sf[5] = Ft[1];
@@ -407,7 +429,7 @@ __global__ void __launch_bounds__(kMarchLanes) k_batch_march(BArgs a, const MIte
             const int64_t node = bc.off + X[1 - D2Q9::t.c[q][0]] + Y[1 - D2Q9::t.c[q][1]];
             fn[q] = ldf<S, T>(a.fin, q * a.P + node, wref_of<S, T>(a, bc, node, q));
         }
-        ring[(yr - it.y0 + 1) & 3][t] = potential<T>(a.eos, density<T>(fn), T(bc.T));
+        ring[(yr - it.y0 + 1) & 3][t] = bpotential<T>(a.eos, density<T>(fn), T(bc.T));
         if (MPLB_BATCH_PRIO) __builtin_amdgcn_s_setprio(0);
     };
     row(it.y0 - 1);
@@ -494,7 +516,7 @@ __global__ void k_batch_rho_psi(BArgs a) {
     pull<S, T>(a, bc, p, a.fin, v);
     const T rho = density<T>(v);
     ((T*)a.rho)[p.gid] = rho;
-    ((T*)a.psi)[p.gid] = potential<T>(a.eos, rho, T(bc.T));
+    ((T*)a.psi)[p.gid] = bpotential<T>(a.eos, rho, T(bc.T));
 }
 
 __global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
