@@ -2144,10 +2144,10 @@ int fused_create(mplb_case* c) {
                                m.nz >= 3;
         z->march = periodic3 && !z->thermal;
         // thermal single pass: xuMRT f with myMRT g (the D3Q15 thermal pair).  The default
-        // with FP32 storage (1.285 against 1.383 ms per step for the three passes on the
-        // 256^3 thermal bubble); with FP16 storage the three passes are faster (0.985
-        // against 1.29 ms).  MPLB_THERMAL_MARCH=1 / =0 forces the single pass / the three
-        // passes.
+        // with FP32 storage (1.10 against 1.38 ms per step for the three passes on the
+        // 256^3 thermal bubble); with FP16 storage the three passes are as fast or faster
+        // (FP16S 0.961 against 0.975 ms, FP16N 1.004 against 1.162).
+        // MPLB_THERMAL_MARCH=1 / =0 forces the single pass / the three passes.
         const char* tenv = std::getenv("MPLB_THERMAL_MARCH");
         const bool tpick = tenv ? std::strcmp(tenv, "1") == 0 : !z->half;
         // (it indexes populations in 32 bits: Q x pitch below 2^32)
