@@ -203,6 +203,8 @@ def run_ensemble(args):
         b.init()
         block, xcd = b.tune(args.tune_steps)
         rows = b.launch_info()["march_rows"]
+        if args.persistent and rows:
+            b.set_persistent(True)
         t1 = time.perf_counter()
         nconv, steps, delta, state = b.run_to_steady(thr[prec], check_every=100, max_steps=args.max_steps)
         t2 = time.perf_counter()
@@ -212,6 +214,7 @@ def run_ensemble(args):
         res[prec] = dict(steady=int(nconv), diverged=int((state == 2).sum()), threshold=thr[prec],
                          dissolved=int(sum(1 for k in range(len(ens)) if state[k] == 1 and lap[k] is None)),
                          wall_s=t2 - t0, run_s=t2 - t1, tune_s=t1 - t0, block=block, xcd_order=xcd, rows=rows,
+                         persistent=bool(args.persistent and rows),
                          steps_median=float(np.median(steps)), steps_max=int(steps.max()),
                          mlups=lu / (t2 - t0) / 1e6, lap=lap, state=state)
         b.close()
@@ -248,8 +251,10 @@ def run_ensemble(args):
             "dissolved": r["dissolved"], "laplace_cases": len(both),
             "threshold": r["threshold"], "wall_s": round(r["wall_s"], 2), "tune_s": round(r["tune_s"], 2),
             "mlups": round(r["mlups"], 1), "steps_median": r["steps_median"], "steps_max": r["steps_max"],
-            "launch": {"kernel": "k_batch_march" if r["rows"] else "k_batch_psi + k_batch_collide",
-                       "rows_per_item": r["rows"], "block": r["block"], "xcd_order": r["xcd_order"]},
+            "launch": {"kernel": ("k_batch_persist" if r["persistent"] else "k_batch_march") if r["rows"]
+                       else "k_batch_psi + k_batch_collide",
+                       "rows_per_item": r["rows"], "block": r["block"], "xcd_order": r["xcd_order"],
+                       "persistent": r["persistent"]},
             "laplace_err_vs_fp64": e, "laplace_fit": fits,
             "laplace_fit_max_rel_err_vs_fp64": max([f["rel_err_vs_fp64"] for f in fits], default=None),
         }
@@ -844,6 +849,8 @@ def main():
     ap.add_argument("--steady-fp16n", type=float, default=1e-4)
     ap.add_argument("--max-steps", type=int, default=200000)
     ap.add_argument("--tune-steps", type=int, default=20)
+    ap.add_argument("--persistent", action="store_true",
+                    help="ensemble: each 100-step run as one persistent cooperative launch of the single pass")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM traffic counters)")
     ap.add_argument("--pattern-probe", action="store_true",

@@ -218,6 +218,13 @@ int mplb_batch_tune(mplb_batch* b, int steps_per_trial, int* chosen_block, int* 
 int mplb_batch_set_launch(mplb_batch* b, int block, int xcd_order);
 int mplb_batch_set_march(mplb_batch* b, int rows);
 int mplb_batch_march_rows(const mplb_batch* b);
+/* on != 0: runs of steps (mplb_batch_step, each graph run of mplb_batch_run_to_steady)
+ * go out as one persistent cooperative launch of the single-pass kernel, the grid
+ * meeting at a barrier between steps, instead of one launch per step; results are
+ * identical.  Needs the single-pass kernel (rows > 0; mplb_batch_tune may pick the
+ * two-pass kernels, which then run per step).  No counterpart in the reference (its
+ * cases run one multiPhasePP process each, multiPhasePP.c:157-241). */
+int mplb_batch_set_persistent(mplb_batch* b, int on);
 int mplb_batch_launch_info(const mplb_batch* b, int* block, int* xcd_order, int* workgroups, int64_t* nodes);
 
 /* ---------------------------------------------------------------------------------

@@ -61,7 +61,7 @@ EXPORTS = [
     "mplb_batch_create", "mplb_batch_close", "mplb_batch_last_error", "mplb_batch_set_case",
     "mplb_batch_init", "mplb_batch_step", "mplb_batch_run_to_steady", "mplb_batch_get_field",
     "mplb_batch_case_steps", "mplb_batch_tune", "mplb_batch_set_launch", "mplb_batch_launch_info",
-    "mplb_batch_set_march", "mplb_batch_march_rows",
+    "mplb_batch_set_march", "mplb_batch_march_rows", "mplb_batch_set_persistent",
     "mplb_density_change", "mplb_set_radial_density", "mplb_set_radial_density_noalloc",
     "mplb_write_lattice", "mplb_write_fields", "mplb_read_field",
     "mplb_mlp_create", "mplb_mlp_close", "mplb_mlp_last_error", "mplb_mlp_num_params", "mplb_mlp_set_params",
@@ -119,6 +119,7 @@ def library():
         "mplb_batch_set_launch": ([vp, i, i], i),
         "mplb_batch_set_march": ([vp, i], i),
         "mplb_batch_march_rows": ([vp], i),
+        "mplb_batch_set_persistent": ([vp, i], i),
         "mplb_batch_launch_info": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(C.c_int64)], i),
         "mplb_density_change": ([vp, C.POINTER(d)], i),
         "mplb_write_lattice": ([vp, cp], i),
@@ -408,6 +409,10 @@ class Batch:
     def set_march(self, rows):
         """Single-pass kernel with `rows` rows per work item (0: the two-pass kernels)."""
         self._check(self.L.mplb_batch_set_march(self.h, rows))
+
+    def set_persistent(self, on=True):
+        """Runs of steps as one persistent cooperative launch of the single-pass kernel."""
+        self._check(self.L.mplb_batch_set_persistent(self.h, 1 if on else 0))
 
     def launch_info(self):
         b, x, w, n = C.c_int(), C.c_int(), C.c_int(), C.c_int64()

@@ -21,6 +21,7 @@
 // reference restatement), FP32, or FP16 with FP32 arithmetic: deviations f - w_k rho_ref
 // (FP16S) or relative deviations (f - w_k R(x)) / (w_k R(x)), R(x) the node's own density
 // when it was written, kept per node in FP16 (FP16N).
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -401,11 +402,7 @@ __device__ inline int wrap_dup(int u, int n) { return u < 0 ? u + (n - 1) : (u >
 #define MPLB_BATCH_PRIO 2
 #endif
 template <typename S, typename T>
-__global__ void __launch_bounds__(kMarchLanes) k_batch_march(BArgs a, const MItem* items, int per, int nitems) {
-    __shared__ T ring[4][kMarchLanes];
-    const int wi = (int)((blockIdx.x & 7u) * (unsigned)per + (blockIdx.x >> 3));
-    if (wi >= nitems) return;
-    const MItem it = items[wi];
+__device__ inline void march_item(const BArgs& a, const MItem& it, T (*ring)[kMarchLanes]) {
     if (it.ncase == 0) return;
     const int t = threadIdx.x;
     const int nx = it.nx;
@@ -449,6 +446,41 @@ __global__ void __launch_bounds__(kMarchLanes) k_batch_march(BArgs a, const MIte
         }
 #pragma unroll
         for (int q = 0; q < 9; q++) fc[q] = fn[q];
+    }
+}
+
+template <typename S, typename T>
+__global__ void __launch_bounds__(kMarchLanes) k_batch_march(BArgs a, const MItem* items, int per, int nitems) {
+    __shared__ T ring[4][kMarchLanes];
+    const int wi = (int)((blockIdx.x & 7u) * (unsigned)per + (blockIdx.x >> 3));
+    if (wi >= nitems) return;
+    march_item<S, T>(a, items[wi], ring);
+}
+
+// The same steps from one persistent launch (cooperative, so every workgroup is resident
+// at once): each workgroup walks the work items of its XCD slice, item v of the launch
+// order being taken by workgroup v mod G (G, the grid, a multiple of 8, so an item keeps
+// the XCD the per-step launch gives it), and the grid meets at a barrier between steps.
+// The level buffers alternate per step as in the per-step launches: step s reads
+// f0 / r0 when s is even.  Every workgroup runs every step and reaches every barrier.
+template <typename S, typename T>
+__global__ void __launch_bounds__(kMarchLanes) k_batch_persist(BArgs a, const MItem* items, int per, int nitems,
+                                                                int nsteps, void* f0, void* f1, _Float16* r0,
+                                                                _Float16* r1) {
+    __shared__ T ring[4][kMarchLanes];
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const unsigned total = 8u * (unsigned)per;
+    for (int s = 0; s < nsteps; s++) {
+        a.fin = (s & 1) ? f1 : f0;
+        a.fout = (s & 1) ? f0 : f1;
+        a.rin = (s & 1) ? r1 : r0;
+        a.rout = (s & 1) ? r0 : r1;
+        for (unsigned v = blockIdx.x; v < total; v += gridDim.x) {
+            const int wi = (int)((v & 7u) * (unsigned)per + (v >> 3));
+            if (wi < nitems) march_item<S, T>(a, items[wi], ring);
+            __syncthreads();      // the next item's rows reuse the LDS ring
+        }
+        if (s + 1 < nsteps) grid.sync();
     }
 }
 
@@ -559,6 +591,7 @@ struct mplb_batch {
     int mper = 0;
     bool march_ok = true;               // every case is at most kMarchLanes wide
     int mrows = 32;                     // rows per march item; 0: the two-pass kernels
+    bool persist = false;               // single pass: one persistent launch per run of steps
     bool initialized = false;
     bool fields_valid = false;
     int64_t steps = 0;
@@ -710,10 +743,41 @@ void launch_steps(mplb_batch* b, int nsteps) {
     }
 }
 
+// nsteps single-pass steps from one persistent cooperative launch; the grid is the number
+// of workgroups that fit the device at once (from the occupancy query), rounded down to
+// a multiple of 8 and at most the item count, so the launch fails rather than wait on a
+// workgroup that cannot start
+template <typename S, typename T>
+int launch_persist(mplb_batch* b, int nsteps) {
+    if (nsteps <= 0 || b->nmwork <= 0) return 0;
+    int bpc = 0, cus = 0;
+    BCHK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_batch_persist<S, T>, kMarchLanes, 0));
+    BCHK(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->dev));
+    int g = std::min(bpc * cus, b->nmwork) / 8 * 8;
+    if (g < 8) return bfail(b, MPLB_E_HIP, "persistent launch: fewer than 8 resident workgroups");
+    BArgs a = bargs(b);
+    const MItem* items = b->d_mwork;
+    int per = b->mper, nitems = b->nmwork, ns = nsteps;
+    void* f0 = b->fa;
+    void* f1 = b->fb;
+    _Float16* r0 = b->ra;
+    _Float16* r1 = b->rb;
+    void* args[] = {&a, &items, &per, &nitems, &ns, &f0, &f1, &r0, &r1};
+    BCHK(b, hipLaunchCooperativeKernel((const void*)k_batch_persist<S, T>, dim3((unsigned)g), dim3(kMarchLanes), args, 0,
+                                       b->stream));
+    if (nsteps & 1) {
+        std::swap(b->fa, b->fb);
+        std::swap(b->ra, b->rb);
+    }
+    return 0;
+}
+
 // nsteps steps through a captured graph; nsteps is even, so the buffers end where they
-// started and each replay continues from the previous one
+// started and each replay continues from the previous one.  With the persistent kernel,
+// one cooperative launch instead.
 template <typename S, typename T>
 int run_graph(mplb_batch* b, int nsteps) {
+    if (b->persist && b->mrows > 0) return launch_persist<S, T>(b, nsteps);
     if (b->gexec && (b->graph_stale || b->graph_steps != nsteps)) {
         BCHK(b, hipGraphExecDestroy(b->gexec));
         BCHK(b, hipGraphDestroy(b->graph));
@@ -776,6 +840,7 @@ int fields_t(mplb_batch* b) {
 
 template <typename S, typename T>
 int steps_plain(mplb_batch* b, int n) {
+    if (b->persist && b->mrows > 0) return launch_persist<S, T>(b, n);
     launch_steps<S, T>(b, n);
     BCHK(b, hipGetLastError());
     return 0;
@@ -1003,6 +1068,14 @@ int mplb_batch_set_march(mplb_batch* b, int rows) {
 }
 
 int mplb_batch_march_rows(const mplb_batch* b) { return b ? b->mrows : MPLB_E_ARG; }
+
+int mplb_batch_set_persistent(mplb_batch* b, int on) {
+    if (!b) return MPLB_E_ARG;
+    if (on && !b->march_ok)
+        return bfail(b, MPLB_E_MODEL, "the persistent kernel is the single pass: every case at most 256 nodes wide");
+    b->persist = on != 0;
+    return 0;
+}
 
 int mplb_batch_run_to_steady(mplb_batch* b, double threshold, int check_every, int64_t max_steps,
                              int64_t* case_steps, double* case_delta, int* case_state) {

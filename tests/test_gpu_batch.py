@@ -1,8 +1,8 @@
 """Batched ensembles (mplb_batch_*) through the C ABI: every case of a batch must follow
 the reference's own time loop for that case alone, so each is compared with a CPU oracle
 run of the same case (FP64 mode: bit-identical; FP32 and FP16-deviation storage: within
-the stated tolerances), and the graph-captured, compacting and autotuned paths must give
-exactly the plain stepping's result."""
+the stated tolerances), and the graph-captured, compacting, autotuned and persistent
+paths must give exactly the plain stepping's result."""
 import numpy as np
 import pytest
 
@@ -199,3 +199,32 @@ def test_single_pass_equals_two_pass(precision):
         for k in range(len(cases)):
             np.testing.assert_array_equal(one.get(k, "rho"), two.get(k, "rho"))
             np.testing.assert_array_equal(one.get(k, "p"), two.get(k, "p"))
+
+
+@pytest.mark.parametrize("precision", [M.FP64, M.FP32, M.FP16N])
+def test_persistent_launch_equals_per_step_launches(precision):
+    """One persistent cooperative launch per run of steps (the grid meets at a barrier
+    between steps) against one launch per step: identical states after an odd number of
+    plain steps (the level buffers end swapped), after further graph-length runs, and
+    through run_to_steady; four cases per work item, and (one row per item) about 1360
+    work items, more than the resident workgroups, so each workgroup takes several."""
+    cases = SMALL + [(32, 32, 0.65 + 0.25 * (k % 11) / 10, 5.0 + (k % 4), k % 2 == 0) for k in range(160)]
+    runs = []
+    for persistent in (False, True):
+        b, _, _ = make_batch(cases, precision)
+        b.set_march(1)
+        if persistent:
+            b.set_persistent(True)
+        b.init()
+        b.step(37)
+        after_plain = [b.get(k, "rho") for k in range(len(cases))]
+        nconv, steps, delta, state = b.run_to_steady(1e-12, check_every=40, max_steps=160)
+        runs.append((after_plain, [b.get(k, "rho") for k in range(len(cases))],
+                     [b.get(k, "p") for k in range(len(cases))], steps, state))
+    (p0, r0, q0, s0, st0), (p1, r1, q1, s1, st1) = runs
+    np.testing.assert_array_equal(s0, s1)
+    np.testing.assert_array_equal(st0, st1)
+    for k in range(len(cases)):
+        np.testing.assert_array_equal(p1[k], p0[k])
+        np.testing.assert_array_equal(r1[k], r0[k])
+        np.testing.assert_array_equal(q1[k], q0[k])
