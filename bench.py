@@ -9,7 +9,11 @@ D3Q15 xuMRT scheme); --thermal adds the myMRT energy field with heat transfer.
 
 One "step" is one multiPhasePP time step over the whole lattice; the metric is MLUPS
 = stored nodes x steps / time / 1e6, timed with HIP events on the library's stream
-after W warm-up steps, inputs resident in HBM.  Prints one JSON line.
+after W warm-up steps, inputs resident in HBM.  Prints one JSON line, which also
+carries sub-records timed in the same run: `thermal` (the same bubble with the myMRT
+energy field and heat transfer), `ensemble` (the batched D2Q9 ensemble step on 256
+cases, FP32, 2000 steps), `copy_ceiling` (flat copies of the same bytes) and
+`cpu_baseline` (the FP64 oracle on the host cores).
 
 Other workloads (--workload):
   ensemble   1024 D2Q9 liMRT droplet/bubble cases run to steady state in one batch per
@@ -171,6 +175,41 @@ def laplace(rho, p, L, rho_v, rho_l):
     inner = r < max(1.5, 0.5 * R)
     rin, pin = float(rho[inner].mean()), float(p[inner].mean())
     return dict(rho_in=rin, rho_out=rout, dp=pin - pout, R=R, gamma=(pin - pout) * R)
+
+
+def ensemble_record(args, ncases=256, steps=2000):
+    """The batched-ensemble step (the C3 kernel) timed inside the default run: the first
+    ncases cases of the seed-7 ensemble in one FP32 batch, launch shape from the
+    autotuner, `steps` steps in graph-captured 100-step runs with no steady criterion (no
+    case leaves the table).  MLUPS of all case steps."""
+    import multiphaselb_amd as M
+    from multiphaselb_amd.casefiles import CaseSpec
+    from multiphaselb_amd.initfields import ensemble, ensemble_density
+
+    ens = ensemble(ncases, seed=7)
+    hosts = {}
+    for c in ens:
+        if c["L"] not in hosts:
+            d = tempfile.mkdtemp(prefix="mplb_ens_")
+            M.write_case(d, CaseSpec(lbmodel=0, nx=c["L"], ny=c["L"], btype=3, f_model=0, a=0.5, b=4.0,
+                                     end=10 ** 8, write_interval=10 ** 8))
+            hosts[c["L"]] = M.Case(d, heat_transfer=False, device=-1)
+    b = M.Batch([hosts[c["L"]] for c in ens], [ensemble_density(c) for c in ens], [c["T"] for c in ens],
+                precision=M.FP32)
+    b.init()
+    block, xcd = b.tune(args.tune_steps)
+    rows = b.launch_info()["march_rows"]
+    s0 = np.array([b.case_steps(k) for k in range(len(ens))])
+    t0 = time.perf_counter()
+    nconv, s1, delta, state = b.run_to_steady(0.0, check_every=100, max_steps=steps)
+    t1 = time.perf_counter()
+    lu = float(sum(c["L"] ** 2 * int(s1[k] - s0[k]) for k, c in enumerate(ens)))
+    b.close()
+    return {"value": round(lu / (t1 - t0) / 1e6, 1), "unit": "MLUPS", "cases": len(ens),
+            "nodes": int(sum(c["L"] ** 2 for c in ens)), "steps": steps, "precision": "fp32",
+            "wall_s": round(t1 - t0, 3), "finite": bool((state != 2).all()),
+            "launch": {"kernel": "k_batch_march" if rows else "k_batch_psi + k_batch_collide",
+                       "rows_per_item": rows, "block": block, "xcd_order": xcd}}
 
 
 def run_ensemble(args):
@@ -822,6 +861,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-thermal-record", action="store_true",
                     help="default workload: skip the thermal sub-record")
+    ap.add_argument("--no-ensemble-record", action="store_true",
+                    help="default workload: skip the batched-ensemble sub-record")
     ap.add_argument("--workload", default="bubble3d", choices=["bubble3d", "ensemble", "warmstart", "accuracy"])
     ap.add_argument("--acc-size", type=int, default=256, help="accuracy: bubble edge (the column is n/2 x n/2 x 3n/2)")
     ap.add_argument("--acc-c2-steps", type=int, default=5000, help="accuracy: column steps compared")
@@ -878,6 +919,9 @@ def main():
         t = measure(args, True)
         out["thermal"] = {k: t[k] for k in ("value", "unit", "ms_per_step", "config", "hbm_gbs_effective",
                                             "step_roofline_frac", "roofline", "kernels_ms", "finite")}
+    if not args.thermal and not args.no_ensemble_record and args.precision != "fp64":
+        # the batched-ensemble step of C3, timed the same way (a short fixed-step run)
+        out["ensemble"] = ensemble_record(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
     if out["roofline"]["achieved"]:
