@@ -205,9 +205,14 @@ def ensemble_record(args, ncases=256, steps=2000):
     t1 = time.perf_counter()
     lu = float(sum(c["L"] ** 2 * int(s1[k] - s0[k]) for k, c in enumerate(ens)))
     b.close()
-    return {"value": round(lu / (t1 - t0) / 1e6, 1), "unit": "MLUPS", "cases": len(ens),
+    mlups = lu / (t1 - t0) / 1e6
+    # populations read once and written once per node-step: 2 x 9 x 4 B
+    gbs = mlups * 1e6 * 72 / 1e9
+    return {"value": round(mlups, 1), "unit": "MLUPS", "cases": len(ens),
             "nodes": int(sum(c["L"] ** 2 for c in ens)), "steps": steps, "precision": "fp32",
             "wall_s": round(t1 - t0, 3), "finite": bool((state != 2).all()),
+            "hbm_gbs_effective": round(gbs, 1), "step_roofline_frac": round(gbs / PEAK_HBM_GBS, 4),
+            "bytes_per_lu": 72,
             "launch": {"kernel": "k_batch_march" if rows else "k_batch_psi + k_batch_collide",
                        "rows_per_item": rows, "block": block, "xcd_order": xcd}}
 
