@@ -1108,9 +1108,9 @@ k_fused_march_pf(FArgs a) {
 //      the previous step's T and U there (LDS rings of three planes),
 //   2. at plane z + 1, on the column and a one-site ring: heat source from the rho, T and U
 //      stencils in LDS, g pulled for its sum, new temperature and psi (LDS ring),
-//   3. collides plane z: f pulled two planes earlier (registers), psi stencil from LDS,
-//      force and velocity, f collision; g pulled again (an L2 hit: it was pulled one plane
-//      earlier), g collision; writes both, the new T and U.
+//   3. collides plane z: f pulled two planes earlier (kept in LDS), psi stencil from LDS,
+//      force and velocity, f collision; the own site's g pulled one plane earlier (kept in
+//      registers), g collision; writes both, the new T and U.
 // Per node and step HBM sees f and g read once and written once and T, U read once and
 // written once.  The arithmetic of every node is that of k_fused_density, k_fused_heat
 // and k_fused_collide (shared device functions); the compiler may still fuse a
