@@ -1117,8 +1117,9 @@ k_fused_march_pf(FArgs a) {
 // multiply-add differently in the two kernels, so results agree to FP32 rounding.
 // Column of TMX x TMY sites, one thread each (64 x 8: 512 threads; the rings around the
 // column hold 304 and 148 sites, one extra site for the first threads).  On the 256^3
-// thermal bubble 64 x 8 ran at 1.48 ms per step, 32 x 8 at 1.50, 32 x 16 at 1.53 and 64 x 4
-// (three f pulls per thread and plane) at 1.86.
+// thermal bubble this kernel runs 64 x 8 at 1.095 ms per step, 32 x 16 at 1.174 and 32 x 8
+// (256 threads, two workgroups per CU) at 1.437; 64 x 4 (three f pulls per thread and
+// plane) ran at 1.86 against 1.48 for 64 x 8 in an earlier form of it.
 #ifndef MPLB_TMX
 #define MPLB_TMX 64
 #define MPLB_TMY 8
