@@ -834,9 +834,9 @@ __device__ inline float stored_pull(raw_t<S> v, const unsigned* r, int q, const 
 #ifndef MPLB_MARCH_WAVES
 #define MPLB_MARCH_WAVES 3
 #endif
-// FP16 storage (FP16S, FP16N) runs four waves per SIMD (128 VGPRs; FP16S spills six):
-// 0.340 against 0.359 ms per step for FP16S on the 256^3 bubble, 0.434 against 0.437 for
-// FP16N
+// FP16 storage (FP16S, FP16N) runs four waves per SIMD (128 VGPRs, seven spilled): on the
+// 256^3 bubble FP16S's march takes 0.319 against 0.330 ms at three waves (no spills),
+// FP16N's 0.425 at either
 #ifndef MPLB_MARCH_WAVES_H
 #define MPLB_MARCH_WAVES_H 4
 #endif
